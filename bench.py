@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X sorting path.
+
+Default workload (the headline configuration): one batched sort of 4096
+independent int32 arrays x 1024 keys, keys uniform over the full int32 range
+from seed 42, one lane group per array, through the C ABI
+(cs_batched_sort, method auto).  A step is one sort of the whole batch from
+the pristine input buffer into the output buffer (one HBM read + one write,
+the reference's in-place sort moves the same bytes); inputs are resident in
+HBM before the timed region starts.
+
+Prints one JSON line.  `value` is Gkeys/s of the whole job; `roofline`
+prices the batched-sort kernel against MI355X's 8 TB/s HBM with the
+algorithmic 8 bytes per int32 key (read + write); `cpu_baseline` is the
+oracle's per-array std::sort over all host cores on the same input.
+
+Other workloads (--workload) print their own lines for the other
+configurations; the default run is the one the headline metric is quoted on.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-sort_amd"))
+
+METRIC = "Gkeys/s & %HBM roofline, batched int32 (fixed vs ragged); serve p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="batched_i32",
+                    choices=["batched_i32", "batched_rec32"])
+    ap.add_argument("--arrays", type=int, default=4096)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--method", default="auto")
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-trials", type=int, default=7)
+    return ap.parse_args()
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy-kernel calibration), committed under
+    profiles/ as JSON; None when absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def bench_batched(args, dtype_name: str):
+    import torch
+    import cusort_hip as cs
+
+    dt = cs.DTYPE_NAMES[dtype_name]
+    method = cs.METHODS.index(args.method)
+    n, arrays = args.n, args.arrays
+    count = n * arrays
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.Stream(device=dev)       # default priority
+    src = cs.device_buffer(dt, count)
+    dst = cs.device_buffer(dt, count)
+    seed = 42
+    with torch.cuda.stream(stream):
+        cs.generate(dt, cs.D_UNIFORM, n, count, seed, src.data_ptr(), stream.cuda_stream)
+    stream.synchronize()
+
+    def step():
+        cs.batched_sort(dt, src.data_ptr(), dst.data_ptr(), arrays, n, method, args.threads, 0,
+                        stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+
+    # correctness of the measured output (outside the timed region)
+    res = cs.to_numpy(dst, dt, count)
+    ok = None
+    cpu = None
+    bytes_per_key = 8 if dt in (cs.I32, cs.U32, cs.F32) else 16
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        host_in = oracle.generate(dt, oracle.D_UNIFORM, n, count, seed)
+        ok = oracle.validate_batched(dt, host_in, res, n)[0] == 0
+        secs, threads = oracle.time_batched(dt, host_in, n, threads=0, trials=args.cpu_trials)
+        cpu = {"value": count / secs / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
+               "sample": f"the full {arrays}x{n} batch, per-array std::sort over {threads} host threads, "
+                         f"median of {args.cpu_trials} trials"}
+    keys_per_s = count / (ms * 1e-3)
+    alg_bytes = count * bytes_per_key
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    name = f"batched_{dtype_name}_{arrays}x{n}"
+    line = {
+        "metric": METRIC,
+        "value": keys_per_s / 1e9,
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32" if dt == cs.I32 else dtype_name,
+        "data": "synthetic",
+        "config": {"workload": f"batched sort, {arrays} arrays x {n} {dtype_name} keys, uniform full range, "
+                               f"seed {seed}, method {args.method}",
+                   "arrays": arrays, "n": n, "method": args.method, "parallelism": "one GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(name)},
+        "cpu_baseline": cpu,
+        "validated": ok,
+    }
+    return line
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(every array is independent: N GPUs would each run this workload on their own share)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "batched_i32":
+        line = bench_batched(args, "i32")
+    else:
+        line = bench_batched(args, "rec32")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+// Batched-sort kernel registry: which (dtype, W, E) tile kernels exist and
+// how to launch them.  Each dtype's kernels are instantiated in their own
+// translation unit (batched_<dtype>.hip) so they compile in parallel.
+#pragma once
+
+#include "bitonic_lds.hpp"
+#include "bitonic_tile.hpp"
+
+namespace cs {
+
+using LaunchFn = hipError_t (*)(const void* in, void* out, int64_t arrays, int n, int logw, unsigned flags,
+                                int threads, hipStream_t stream);
+
+struct KernelInfo {
+    LaunchFn launch = nullptr;
+    const void* kernel = nullptr;  // device function, for hipFuncGetAttributes
+    int block = 0;                 // threads per workgroup (0: chosen at launch)
+    int items = 0;                 // elements per lane
+    int apb = 0;                   // arrays per workgroup
+    int lds = 0;                   // static LDS bytes
+};
+
+constexpr int kMaxLogW4 = 14;   // 16384 4-byte keys
+constexpr int kMaxLogW8 = 13;   // 8192 8-byte keys / IntAndData records
+constexpr int kMaxLogW16 = 12;  // 4096 16-byte records
+
+constexpr int max_logw(int dtype) {
+    return (dtype == CS_I32 || dtype == CS_U32 || dtype == CS_F32) ? kMaxLogW4
+           : (dtype == CS_REC_I64_I32) ? kMaxLogW16 : kMaxLogW8;
+}
+
+// Default elements per lane of the register tile for W = 2^logw.
+constexpr int auto_loge(int dtype, int logw) {
+    (void)dtype;
+    return logw < 4 ? logw : 4;
+}
+
+template <class E>
+constexpr int tile_lds_bytes(int logw, int loge) {
+    const int T = 1 << (logw - loge);
+    const int block = T >= 256 ? T : 256;
+    const int apb = block / T;
+    const int ntrans = make_plan(logw, loge).ntrans;
+    const int padshift = sizeof(typename E::Key) == 4 ? 5 : 4;
+    const int region = (1 << logw) + ((1 << logw) >> padshift);
+    const int nbuf = T > 64 ? 2 : 1;
+    return ntrans ? nbuf * apb * region * (int(sizeof(typename E::Key)) + (E::kPayload ? 4 : 0)) : 0;
+}
+
+template <int DT>
+constexpr bool tile_exists(int logw, int loge) {
+    if (loge < 0 || loge > logw || logw > max_logw(DT)) return false;
+    if (logw > 0 && loge == 0) return false;                    // needs at least one slot bit
+    if (logw - loge > 10) return false;                        // at most 1024 lanes
+    if (tile_lds_bytes<Elem<DT>>(logw, loge) > 160 * 1024) return false;
+    // The reference's own types get every v6 tiling (items 1..16); the others
+    // only the tuned default.
+    if (DT == CS_I32 || DT == CS_REC_I32_I32) return loge <= 4;
+    return loge == auto_loge(DT, logw);
+}
+
+template <int DT, int LW, int LE>
+hipError_t launch_tile(const void* in, void* out, int64_t arrays, int n, int, unsigned, int, hipStream_t s) {
+    using C = TileCfg<Elem<DT>, LW, LE>;
+    using S = typename Elem<DT>::Storage;
+    const int64_t blocks = (arrays + C::APB - 1) / C::APB;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_bitonic_tile<Elem<DT>, LW, LE>), dim3(unsigned(blocks)), dim3(C::BLOCK), 0, s,
+                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n);
+    return hipGetLastError();
+}
+
+template <int DT>
+hipError_t launch_lds(const void* in, void* out, int64_t arrays, int n, int logw, unsigned flags, int threads,
+                      hipStream_t s) {
+    using E = Elem<DT>;
+    using S = typename E::Storage;
+    if (arrays <= 0) return hipSuccess;
+    const size_t bytes = (size_t(1) << logw) * (sizeof(typename E::Key) + (E::kPayload ? 4 : 0));
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bitonic_lds<E>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_bitonic_lds<E>, dim3(unsigned(arrays)), dim3(unsigned(threads)), bytes, s,
+                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n, logw, flags);
+    return hipGetLastError();
+}
+
+template <int I, int N, class F>
+inline void hfor(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        hfor<I + 1, N>(f);
+    }
+}
+
+template <int DT>
+KernelInfo tile_lookup(int logw, int loge) {
+    KernelInfo r{};
+    hfor<0, max_logw(DT) + 1>([&](auto w_) {
+        constexpr int LW = decltype(w_)::value;
+        hfor<0, 6>([&](auto e_) {
+            constexpr int LE = decltype(e_)::value;
+            if constexpr (tile_exists<DT>(LW, LE)) {
+                if (LW == logw && LE == loge) {
+                    using C = TileCfg<Elem<DT>, LW, LE>;
+                    r.launch = &launch_tile<DT, LW, LE>;
+                    r.kernel = reinterpret_cast<const void*>(&k_bitonic_tile<Elem<DT>, LW, LE>);
+                    r.block = C::BLOCK;
+                    r.items = C::NE;
+                    r.apb = C::APB;
+                    r.lds = C::LDS_BYTES;
+                }
+            }
+        });
+    });
+    return r;
+}
+
+template <int DT>
+KernelInfo lds_lookup() {
+    KernelInfo r{};
+    r.launch = &launch_lds<DT>;
+    r.kernel = reinterpret_cast<const void*>(&k_bitonic_lds<Elem<DT>>);
+    r.apb = 1;
+    return r;
+}
+
+// Defined in batched_<dtype>.hip.
+KernelInfo tile_kernel(int dtype, int logw, int loge);
+KernelInfo lds_kernel(int dtype);
+
+KernelInfo tile_kernel_i32(int, int);
+KernelInfo tile_kernel_u32(int, int);
+KernelInfo tile_kernel_f32(int, int);
+KernelInfo tile_kernel_u64(int, int);
+KernelInfo tile_kernel_i64(int, int);
+KernelInfo tile_kernel_rec32(int, int);
+KernelInfo tile_kernel_rec64(int, int);
+KernelInfo lds_kernel_i32();
+KernelInfo lds_kernel_u32();
+KernelInfo lds_kernel_f32();
+KernelInfo lds_kernel_u64();
+KernelInfo lds_kernel_i64();
+KernelInfo lds_kernel_rec32();
+KernelInfo lds_kernel_rec64();
+
+}  // namespace cs

@@ -1,0 +1,280 @@
+// Register-tile bitonic sort for gfx950 (wave64): one array per lane group
+// of T = W / E lanes (several arrays per wave when T < 64, one workgroup per
+// array when T > 64), E elements per lane held in VGPRs.
+//
+// Network: the all-ascending "reflection" bitonic network of the reference
+// (cusort.hpp:545-547, 686-697; cusort_v6.hpp:38-128): stage k first
+// compares i with i ^ (2^k - 1), then i with i ^ 2^j for j = k-2 .. 0, the
+// lower index keeping the minimum and each side keeping its own element on
+// equal keys.  What is new is how the network is mapped onto the hardware.
+//
+// Layouts.  The W = 2^w positions of an array are spread over register slot
+// bits and lane bits.  Every layout used here is L(h): the e = log2(E) slot
+// bits hold logical bits h-e+1 .. h, the lane bits hold the other logical
+// bits in increasing order.  A compare-exchange on logical bit j is a pair
+// of VGPRs (one v_min + one v_max per pair, no cross-lane traffic) whenever
+// j is a slot bit of the current layout.  Stages k <= e run entirely in the
+// identity layout L(e-1).  A stage k > e is cut into chunks of at most e bits
+// from the top (the first chunk takes the remainder so the last chunk is
+// always bits e-1..0), and the tile is re-laid between chunks by one LDS
+// transpose (write every VGPR at its logical position, read back in the next
+// layout).  So every one of the w(w+1)/2 layers costs one VALU op per element
+// and the cross-lane work is moved to the LDS pipe: sum over k>e of
+// ceil(k/e) transposes.
+//
+// The reflection of stage k > e is folded into the transpose that opens the
+// stage: the upper half (bit k-1 = 1) is read back with logical bits
+// 0..k-2 flipped, which turns the reflection into a plain compare on bit
+// k-1; while the flip is in force the upper half merges descending (bit k-1
+// is a slot bit there, so the direction is a compile-time property of each
+// VGPR); the next transpose undoes the flip as it writes.  Position by
+// position the result is the reference network's, so its padding and tie
+// guarantees carry over: max-key padding placed at logical positions >= n
+// stays there (it never moves to a lower logical index), records are never
+// duplicated or lost.
+#pragma once
+
+#include <type_traits>
+
+#include "elem.hpp"
+#include "plan.hpp"
+
+namespace cs {
+
+template <int I, int N, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        sfor<I + 1, N>(f);
+    }
+}
+
+// --------------------------------------------------------------- config --
+template <class E, int LOGW, int LOGE> struct TileCfg {
+    using Key = typename E::Key;
+    using Storage = typename E::Storage;
+    static constexpr int W = 1 << LOGW;
+    static constexpr int NE = 1 << LOGE;             // elements per lane
+    static constexpr int LOGT = LOGW - LOGE;
+    static constexpr int T = 1 << LOGT;              // lanes per array
+    static constexpr int BLOCK = T >= 256 ? T : 256; // threads per workgroup
+    static constexpr int APB = BLOCK / T;            // arrays per workgroup
+    static constexpr bool kMultiWave = T > 64;
+    static constexpr int NTRANS = PlanOf<LOGW, LOGE>::value.ntrans;
+    // LDS address of logical position x: x + (x >> PADSHIFT) keeps a
+    // column of lanes 2^PADSHIFT elements apart on distinct banks.
+    static constexpr int PADSHIFT = sizeof(Key) == 4 ? 5 : 4;
+    static constexpr int REGION = W + (W >> PADSHIFT);
+    static constexpr int NBUF = kMultiWave ? 2 : 1;
+    static constexpr int LDS_KEYS = NTRANS ? NBUF * APB * REGION : 1;
+    static constexpr int LDS_PAYLOAD = (NTRANS && E::kPayload) ? NBUF * APB * REGION : 1;
+    static constexpr int LDS_BYTES = NTRANS ? LDS_KEYS * int(sizeof(Key)) + (E::kPayload ? LDS_PAYLOAD * 4 : 0) : 0;
+    static_assert(LOGE <= LOGW, "more elements per lane than the array holds");
+    static_assert(T <= 1024, "at most 1024 lanes per array");
+    static_assert(LDS_BYTES <= 160 * 1024, "tile exceeds the 160 KB LDS of a gfx950 workgroup");
+};
+
+template <class E, int LOGW, int LOGE>
+struct Tile {
+    using C = TileCfg<E, LOGW, LOGE>;
+    using Key = typename E::Key;
+    static constexpr int NE = C::NE;
+    static constexpr int e = LOGE;
+    static constexpr int w = LOGW;
+
+    Key k[NE];
+    int32_t p[NE];
+
+    template <bool kDesc>
+    __device__ __forceinline__ void cx(int a, int b) { cmpx<E, kDesc>(k[a], p[a], k[b], p[b]); }
+
+    template <int I>
+    __device__ __forceinline__ void run(int t, Key* lk, int32_t* lp) {
+        if constexpr (I < PlanOf<LOGW, LOGE>::value.nops) {
+            constexpr Op op = PlanOf<LOGW, LOGE>::value.ops[I];
+            if constexpr (op.kind == OP_REFLECT) {
+                constexpr int mask = (2 << op.bit) - 1;
+                sfor<0, NE>([&](auto r_) {
+                    constexpr int r = decltype(r_)::value;
+                    if constexpr (((r >> op.bit) & 1) == 0) cx<false>(r, r ^ mask);
+                });
+            } else if constexpr (op.kind == OP_CE) {
+                constexpr int s = op.bit - (op.cur_h - e + 1);
+                static_assert(s >= 0 && s < e, "compare on a lane bit");
+                constexpr int sd = op.dirbit >= 0 ? op.dirbit - (op.cur_h - e + 1) : -1;
+                sfor<0, NE>([&](auto r_) {
+                    constexpr int r = decltype(r_)::value;
+                    if constexpr (((r >> s) & 1) == 0) {
+                        constexpr bool desc = sd >= 0 && ((r >> (sd >= 0 ? sd : 0)) & 1);
+                        cx<desc>(r, r | (1 << s));
+                    }
+                });
+            } else {
+                transpose<op.from_h, op.to_h, op.flip_read, op.flip_write, op.index>(t, lk, lp);
+            }
+            run<I + 1>(t, lk, lp);
+        }
+    }
+
+    template <int FH, int TH, int FR, int FW, int IDX>
+    __device__ __forceinline__ void transpose(int t, Key* lk, int32_t* lp) {
+        static_assert(FW < 0 || FW == FH, "write flip must be the source layout's top bit");
+        static_assert(FR < 0 || FR == TH, "read flip must be the target layout's top bit");
+        constexpr int PS = C::PADSHIFT;
+        constexpr int buf = C::kMultiWave ? (IDX & 1) * C::APB * C::REGION : 0;
+        // Lane parts of the addresses (runtime, shared by all slots); the
+        // slot parts are compile-time offsets (lds_addr is additive over
+        // disjoint bits).
+        const int wa = lds_addr(transpose_base(t, FH, e, false), PS) + buf;
+        const int ra = lds_addr(transpose_base(t, TH, e, false), PS) + buf;
+        const int waf = FW >= 0 ? lds_addr(transpose_base(t, FH, e, true), PS) + buf : 0;
+        const int raf = FR >= 0 ? lds_addr(transpose_base(t, TH, e, true), PS) + buf : 0;
+        sfor<0, NE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            constexpr bool flip = slot_flipped(r, e, FW >= 0);
+            constexpr int off = lds_addr(transpose_slot(r, FH, e, FW >= 0), PS);
+            const int a = (flip ? waf : wa) + off;
+            lk[a] = k[r];
+            if constexpr (E::kPayload) lp[a] = p[r];
+        });
+        if constexpr (C::kMultiWave) {
+            __syncthreads();
+        } else {
+            // One wave owns the region: DS instructions of a wave execute in
+            // order, so only the compiler has to be kept from reordering.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        sfor<0, NE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            constexpr bool flip = slot_flipped(r, e, FR >= 0);
+            constexpr int off = lds_addr(transpose_slot(r, TH, e, FR >= 0), PS);
+            const int a = (flip ? raf : ra) + off;
+            k[r] = lk[a];
+            if constexpr (E::kPayload) p[r] = lp[a];
+        });
+        if constexpr (!C::kMultiWave) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+};
+
+// ------------------------------------------------------------ the kernel --
+// Global I/O.  Keys without payload may be loaded in any order (the sorted
+// output does not depend on it, and max-key padding cannot enter the valid
+// prefix of a key-only sort), so they are read with lanes striding over
+// consecutive 16-byte chunks.  Records are loaded position for position
+// (logical = input index) so the padding argument above applies.
+template <class E, int LOGW, int LOGE>
+__global__ void __launch_bounds__((TileCfg<E, LOGW, LOGE>::BLOCK))
+k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out,
+               int64_t arrays, int n) {
+    using C = TileCfg<E, LOGW, LOGE>;
+    using Storage = typename E::Storage;
+    using Key = typename E::Key;
+    constexpr int NE = C::NE;
+    constexpr int T = C::T;
+    constexpr int V = 16 / int(sizeof(Storage)) > 0 ? 16 / int(sizeof(Storage)) : 1;   // elements per 16 B
+
+    __shared__ Key lds_k[C::LDS_KEYS];
+    __shared__ int32_t lds_p[C::LDS_PAYLOAD];
+
+    const int tid = int(threadIdx.x);
+    const int group = tid >> C::LOGT;
+    const int t = tid & (T - 1);
+    const int64_t a = int64_t(blockIdx.x) * C::APB + group;
+    const bool active = a < arrays;
+    const Storage* src = in + (active ? a : 0) * int64_t(n);
+    Storage* dst = out + (active ? a : 0) * int64_t(n);
+    const int valid = active ? n : 0;
+
+    Tile<E, LOGW, LOGE> tile;
+    // 16-byte accesses need 16-byte aligned array starts (n*sizeof % 16 == 0).
+    const bool vec_in = ((uintptr_t)src & 15) == 0 && V > 1 && NE >= V;
+    const bool vec_out = ((uintptr_t)dst & 15) == 0 && V > 1 && NE >= V;
+
+    // ---- load
+    if constexpr (!E::kPayload) {
+        if (vec_in) {
+            sfor<0, NE / (V > 1 ? V : 1)>([&](auto q_) {
+                constexpr int q = decltype(q_)::value;
+                const int g = (q * T + t) * V;
+                if (g + V <= valid) {
+                    using Vec = __attribute__((ext_vector_type(4))) uint32_t;
+                    const Vec v = *reinterpret_cast<const Vec*>(src + g);
+                    Storage tmp[V];
+                    __builtin_memcpy(tmp, &v, 16);
+                    sfor<0, V>([&](auto j_) {
+                        constexpr int j = decltype(j_)::value;
+                        tile.k[q * V + j] = E::key(tmp[j]);
+                    });
+                } else {
+                    sfor<0, V>([&](auto j_) {
+                        constexpr int j = decltype(j_)::value;
+                        tile.k[q * V + j] = (g + j < valid) ? E::key(src[g + j]) : E::kPad;
+                    });
+                }
+            });
+        } else {
+            sfor<0, NE>([&](auto r_) {
+                constexpr int r = decltype(r_)::value;
+                const int g = r * T + t;
+                tile.k[r] = g < valid ? E::key(src[g]) : E::kPad;
+            });
+        }
+        sfor<0, NE>([&](auto r_) { tile.p[decltype(r_)::value] = 0; });
+    } else {
+        sfor<0, NE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            const int g = t * NE + r;
+            if (g < valid) {
+                const Storage s = src[g];
+                tile.k[r] = E::key(s);
+                tile.p[r] = E::payload(s);
+            } else {
+                tile.k[r] = E::kPad;
+                tile.p[r] = INT32_MAX;
+            }
+        });
+    }
+
+    // ---- network
+    Key* lk = lds_k + (C::NTRANS ? group * C::REGION : 0);
+    int32_t* lp = lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0);
+    tile.template run<0>(t, lk, lp);
+
+    // ---- store: the final layout is L(e-1), lane t holds t*NE .. t*NE+NE-1
+    if (vec_out && !E::kPayload) {
+        sfor<0, NE / (V > 1 ? V : 1)>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            const int g = t * NE + q * V;
+            if (g + V <= valid) {
+                Storage tmp[V];
+                sfor<0, V>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    tmp[j] = E::make(tile.k[q * V + j], 0);
+                });
+                using Vec = __attribute__((ext_vector_type(4))) uint32_t;
+                Vec v;
+                __builtin_memcpy(&v, tmp, 16);
+                *reinterpret_cast<Vec*>(dst + g) = v;
+            } else {
+                sfor<0, V>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    if (g + j < valid) dst[g + j] = E::make(tile.k[q * V + j], 0);
+                });
+            }
+        });
+    } else {
+        sfor<0, NE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            const int g = t * NE + r;
+            if (g < valid) dst[g] = E::make(tile.k[r], tile.p[r]);
+        });
+    }
+}
+
+}  // namespace cs

@@ -1,0 +1,150 @@
+"""Python binding of libcusort_hip.so (the C ABI in include/cusort_hip.h).
+
+The product is the HIP library; this module is the thin host-side mirror the
+tests and the benchmark call through.  PyTorch, when used, only provides
+device memory and streams.  There is no CPU fallback: if the library is
+missing or no GPU is present the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcusort_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cusort_hip.h")
+
+# cs_dtype
+I32, REC_I32_I32, F32, U64, REC_I64_I32, U32, I64 = range(7)
+DTYPE_NAMES = {"i32": I32, "rec32": REC_I32_I32, "f32": F32, "u64": U64, "rec64": REC_I64_I32,
+               "u32": U32, "i64": I64}
+DTYPE_SIZE = {I32: 4, REC_I32_I32: 8, F32: 4, U64: 8, REC_I64_I32: 16, U32: 4, I64: 8}
+
+# cs_method (bench.cu:129-131 names)
+METHODS = ["auto", "bitonic_warp", "bitonic_v1", "bitonic_v2", "bitonic_v3", "bitonic_v4_xor",
+           "bitonic_v5_static", "bitonic_v6_register"]
+AUTO, WARP, V1, V2, V3, V4, V5, V6 = range(8)
+
+FLAG_TESTSORTED, FLAG_FULLSM, FLAG_ASYNC = 1, 2, 4
+
+OK = 0
+E_INVALID_VALUE, E_UNSUPPORTED, E_CAPACITY, E_NO_DEVICE, E_BUDGET, E_AGAIN, E_WORKSPACE = (
+    -1, -2, -3, -4, -5, -6, -7)
+
+(D_UNIFORM, D_SORTED, D_REVERSED, D_CONSTANT, D_DISTINCT16, D_PERTURBED, D_ORGAN_PIPE,
+ D_ALT_EXTREMES, D_DUP50, D_EXTREMES6, D_RANDOM1M, D_BINARY, D_NORMAL_SPECIAL,
+ D_PRESORTED_RUNS) = range(14)
+
+
+class CsError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {strerror(status)} ({status})")
+
+
+_lib = None
+
+
+def build(jobs: int = 8) -> None:
+    """Compile the HIP sources for gfx950 into libcusort_hip.so (make)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run build() (make -C cuda-sort_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, u32, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64
+        P32 = ctypes.POINTER(ctypes.c_int32)
+        L.cs_strerror.restype = ctypes.c_char_p
+        L.cs_strerror.argtypes = [ctypes.c_int]
+        L.cs_version.restype = ctypes.c_int
+        L.cs_dtype_size.restype = ctypes.c_size_t
+        L.cs_dtype_size.argtypes = [ctypes.c_int]
+        L.cs_method_from_name.restype = ctypes.c_int
+        L.cs_method_from_name.argtypes = [ctypes.c_char_p]
+        L.cs_batched_sort.restype = ctypes.c_int
+        L.cs_batched_sort.argtypes = [ctypes.c_int, vp, vp, i64, i32, ctypes.c_int, i32, u32, vp]
+        L.cs_batched_max_n.restype = i32
+        L.cs_batched_max_n.argtypes = [ctypes.c_int]
+        L.cs_batched_describe.restype = ctypes.c_int
+        L.cs_batched_describe.argtypes = [ctypes.c_int, i32, ctypes.c_int, i32, u32, P32, P32, P32, P32, P32]
+        L.cs_generate.restype = ctypes.c_int
+        L.cs_generate.argtypes = [ctypes.c_int, ctypes.c_int, i64, i64, u64, vp, vp]
+        _lib = L
+    return _lib
+
+
+def header_symbols(path: str = HEADER_PATH) -> list[str]:
+    """Function names declared in the C header (for export checks)."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cs_[a-z0-9_]+)\s*\(", text)))
+
+
+def strerror(status: int) -> str:
+    return lib().cs_strerror(status).decode()
+
+
+def check(status: int, what: str) -> None:
+    if status != OK:
+        raise CsError(status, what)
+
+
+def batched_sort(dtype: int, src: int, dst: int, arrays: int, n: int, method: int = AUTO,
+                 threads: int = 0, flags: int = 0, stream: int = 0) -> None:
+    """cs_batched_sort on raw device pointers (ints)."""
+    check(lib().cs_batched_sort(dtype, src, dst, arrays, n, method, threads, flags, stream),
+          f"cs_batched_sort(dtype={dtype}, n={n}, method={METHODS[method] if 0 <= method < 8 else method}, "
+          f"threads={threads}, flags={flags})")
+
+
+def batched_status(dtype: int, src: int, dst: int, arrays: int, n: int, method: int = AUTO,
+                   threads: int = 0, flags: int = 0, stream: int = 0) -> int:
+    return int(lib().cs_batched_sort(dtype, src, dst, arrays, n, method, threads, flags, stream))
+
+
+def batched_max_n(dtype: int) -> int:
+    return int(lib().cs_batched_max_n(dtype))
+
+
+def describe(dtype: int, n: int, method: int = AUTO, threads: int = 0, flags: int = 0) -> dict:
+    vals = [ctypes.c_int32(0) for _ in range(5)]
+    check(lib().cs_batched_describe(dtype, n, method, threads, flags, *[ctypes.byref(v) for v in vals]),
+          "cs_batched_describe")
+    keys = ["block_threads", "items_per_thread", "lds_bytes", "vgprs", "arrays_per_block"]
+    return {k: int(v.value) for k, v in zip(keys, vals)}
+
+
+def generate(dtype: int, dist: int, n: int, count: int, seed: int, dst: int, stream: int = 0) -> None:
+    check(lib().cs_generate(dtype, dist, n, count, seed & (2**64 - 1), dst, stream), "cs_generate")
+
+
+# ------------------------------------------------------------ torch helpers
+def device_buffer(dtype: int, count: int, device: str = "cuda"):
+    """Raw byte buffer for `count` elements of `dtype` on the GPU (torch)."""
+    import torch
+    return torch.empty(max(1, count * DTYPE_SIZE[dtype]), dtype=torch.uint8, device=device)
+
+
+def to_numpy(buf, dtype: int, count: int):
+    """Copy `count` elements of a device byte buffer to a numpy array."""
+    import numpy as np
+    np_dtype = {I32: np.dtype("<i4"), F32: np.dtype("<u4"), U32: np.dtype("<u4"), U64: np.dtype("<u8"),
+                I64: np.dtype("<i8"),
+                REC_I32_I32: np.dtype([("key", "<i4"), ("payload", "<i4")]),
+                REC_I64_I32: np.dtype([("key", "<i8"), ("payload", "<i4"), ("pad", "<i4")])}[dtype]
+    raw = buf[: count * DTYPE_SIZE[dtype]].cpu().numpy()
+    return raw.view(np_dtype).copy()
+
+
+def from_numpy(arr, device: str = "cuda"):
+    """Upload a numpy array as a device byte buffer."""
+    import numpy as np
+    import torch
+    raw = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+    return torch.from_numpy(raw.copy()).to(device)

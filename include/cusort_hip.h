@@ -1,0 +1,164 @@
+/* cusort-hip: C ABI of the MI355X (gfx950) sorting path.
+ *
+ * The reference (berenger-eu/cuda-sort) is a header-only CUDA C++ template
+ * library with no C ABI; callers include cusort.hpp and launch its kernels.
+ * This header is the boundary a foreign caller binds instead: plain pointers,
+ * sizes and integer codes, stream-ordered on a caller-supplied hipStream_t.
+ * Each entry point names the reference interface it replaces (file:line).
+ *
+ * Conventions
+ *  - All device pointers are HIP device allocations owned by the caller.
+ *  - Every call is asynchronous on `stream` (0 = the null stream) unless
+ *    stated otherwise; errors in arguments are reported before anything is
+ *    enqueued.
+ *  - Return value: CS_OK (0), a negative CS_E_* code for argument/capacity
+ *    errors, or a positive hipError_t from the HIP runtime.
+ *  - Output equals host std::sort of each array (records compared by key;
+ *    the order of equal keys is unspecified, as in the reference whose
+ *    bitonic networks are unstable).
+ */
+#ifndef CUSORT_HIP_H
+#define CUSORT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef HIP_INCLUDE_HIP_HIP_RUNTIME_API_H
+typedef struct ihipStream_t* hipStream_t;
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ element types
+ * CS_I32          int32 key                     (reference `int`)
+ * CS_REC_I32_I32  {int32 key; int32 payload}    (reference IntAndData,
+ *                 intanddata.hpp:9-79: 8 B, compared by key only)
+ * CS_F32          fp32 key ordered by IEEE-754 totalOrder
+ *                 (-NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN)
+ * CS_U64          uint64 key
+ * CS_REC_I64_I32  {int64 key; int32 payload; int32 pad} (16 B, key order)
+ * CS_U32, CS_I64  uint32 / int64 keys
+ */
+enum cs_dtype {
+    CS_I32 = 0,
+    CS_REC_I32_I32 = 1,
+    CS_F32 = 2,
+    CS_U64 = 3,
+    CS_REC_I64_I32 = 4,
+    CS_U32 = 5,
+    CS_I64 = 6
+};
+
+/* Batched method names follow bench's --methods list (bench.cu:129-131). */
+enum cs_method {
+    CS_METHOD_AUTO = 0,          /* library's tuned choice                     */
+    CS_BITONIC_WARP = 1,         /* Sort_1Warp_SM_le32 (cusort.hpp:574-606)     */
+    CS_BITONIC_V1 = 2,           /* Sort_1Block_gt32_d_extsm (749-837)          */
+    CS_BITONIC_V2 = 3,           /* ..._v2 (842-1056)                            */
+    CS_BITONIC_V3 = 4,           /* ..._v3 (1061-1300)                           */
+    CS_BITONIC_V4_XOR = 5,       /* cusort_experiments.hpp:180-228              */
+    CS_BITONIC_V5_STATIC = 6,    /* cusort_experiments.hpp:159-233              */
+    CS_BITONIC_V6_REGISTER = 7   /* cusort_v6.hpp:144-184                        */
+};
+
+/* Flags: the reference's compile-time switches, passed at run time.
+ * TESTSORTED  skip the network when the array is already sorted (output
+ *             unchanged, only time differs; cusort.hpp:394-458)
+ * FULLSM      exchange every layer through LDS instead of registers
+ * ASYNC       stage the global load through LDS (memcpy_async analogue)   */
+#define CS_FLAG_TESTSORTED 1u
+#define CS_FLAG_FULLSM 2u
+#define CS_FLAG_ASYNC 4u
+
+/* Status codes. */
+#define CS_OK 0
+#define CS_E_INVALID_VALUE (-1)     /* null pointer, negative size, bad enum  */
+#define CS_E_UNSUPPORTED (-2)       /* (method, n, threads, flags, dtype) outside the domain */
+#define CS_E_CAPACITY (-3)          /* length beyond the on-chip capacity      */
+#define CS_E_NO_DEVICE (-4)         /* no HIP device / runtime failure at init */
+#define CS_E_BUDGET (-5)            /* request exceeds an admission budget     */
+#define CS_E_AGAIN (-6)             /* resource temporarily exhausted          */
+#define CS_E_WORKSPACE (-7)         /* workspace too small / wrong type        */
+
+const char* cs_strerror(int status);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int cs_version(void);
+/* Size in bytes of one element of `dtype` in memory (0 for an unknown dtype). */
+size_t cs_dtype_size(int dtype);
+/* Map a bench method name ("bitonic_v6_register", "auto", ...) to cs_method;
+ * returns -1 for an unknown name. */
+int cs_method_from_name(const char* name);
+
+/* ----------------------------------------------------------- batched sort
+ * Sort `arrays` independent arrays of `n` elements each, packed back to back
+ * (array a = base + a*n, BENCHMARKS.md:25-29), one workgroup (or a lane group
+ * of one wave) per array.  in == out sorts in place (the reference's
+ * behaviour); otherwise `out` receives the sorted arrays and `in` is left
+ * untouched.
+ *
+ * Replaces the reference's batched launches
+ *   benchmarkSortSmall / benchmarkSortBlock<1|2|3> / benchmarkSortV4Xor /
+ *   benchmarkSortV5Static / benchmarkSortV6Register  (bench.cu:355-393)
+ * launched as kernel<<<arrays, threads, W*sizeof(T)*(v1?1:2)>>>(data, n)
+ * (bench.cu:442-552), with W = max(32, next_pow2(n)) (bench.cu:524).
+ *
+ * `threads` follows the reference's per-method domain (0 = method default):
+ *   bitonic_warp        n <= 32, threads 32
+ *   bitonic_v1, v2      threads == W, W <= 1024
+ *   bitonic_v3, v4_xor  threads a multiple of 32 in [32, min(W, 1024)]
+ *   bitonic_v5_static   threads a power of two in [32, min(W, 1024)]
+ *   bitonic_v6_register threads a power of two in [32, min(W, 1024)],
+ *                       W / threads <= 16
+ *   auto                threads ignored
+ * Outside that domain the call returns CS_E_UNSUPPORTED (the reference
+ * silently skipped unlaunchable configurations, bench.cu:425-440).
+ * Capacity: n <= cs_batched_max_n(dtype) (16384 for 4-byte, 8192 for 8-byte,
+ * 4096 for 16-byte elements: 160 KB of LDS per workgroup on gfx950, against the
+ * reference's fixed 48 KB, cusort.hpp:231-237).
+ */
+int cs_batched_sort(int dtype, const void* in, void* out, int64_t arrays, int32_t n,
+                    int method, int32_t threads, uint32_t flags, hipStream_t stream);
+int32_t cs_batched_max_n(int dtype);
+/* Resources of the kernel cs_batched_sort would launch: threads per
+ * workgroup, items per thread, static LDS bytes, VGPRs, arrays per
+ * workgroup (the reference reports the same through cudaFuncGetAttributes
+ * and the occupancy API, bench.cu:425-440).  Any pointer may be NULL. */
+int cs_batched_describe(int dtype, int32_t n, int method, int32_t threads, uint32_t flags,
+                        int32_t* block_threads, int32_t* items_per_thread, int32_t* lds_bytes,
+                        int32_t* vgprs, int32_t* arrays_per_block);
+
+/* Input distributions of cs_generate (numbering shared with the oracle). */
+enum cs_dist {
+    CS_DIST_UNIFORM = 0,        /* uniform over the full key range            */
+    CS_DIST_SORTED = 1,         /* local position (bench.cu:164-167)          */
+    CS_DIST_REVERSED = 2,       /* n-1-local (bench.cu:168)                   */
+    CS_DIST_CONSTANT = 3,       /* 99 (bench.cu:182)                          */
+    CS_DIST_DISTINCT16 = 4,     /* 16 distinct values incl. INT_MIN/INT_MAX   */
+    CS_DIST_PERTURBED = 5,      /* 4*local + U{-8..8} (PERTURBED_2 spacing)   */
+    CS_DIST_ORGAN_PIPE = 6,     /* min(local, n-1-local)                      */
+    CS_DIST_ALT_EXTREMES = 7,   /* INT_MIN, INT_MAX, INT_MIN, ...             */
+    CS_DIST_DUP50 = 8,          /* second half repeats hashed first-half keys */
+    CS_DIST_EXTREMES6 = 9,      /* {INT_MAX,INT_MIN,0,INT_MAX,-1,1} cycle (bench.cu:184-187) */
+    CS_DIST_RANDOM1M = 10,      /* mixBits(i+1234) % 1000001 (bench.cu:171)  */
+    CS_DIST_BINARY = 11,        /* local % 2 (bench.cu:183)                   */
+    CS_DIST_NORMAL_SPECIAL = 12,/* fp32 ~N(0,1) (Irwin-Hall 12) + 1% NaN/-0/inf */
+    CS_DIST_PRESORTED_RUNS = 13 /* 10% of 4096-blocks ascending runs, rest uniform */
+};
+
+/* ------------------------------------------------------- input generators
+ * Fill count elements with distribution `dist` (same numbering and formulas
+ * as the host oracle: uniform, sorted, reversed, constant, 16 distinct,
+ * perturbed, organ pipe, alternating extremes, 50% duplicates, the
+ * reference's EXTREMES / RANDOM / BINARY (bench.cu:152-192), fp32
+ * N(0,1)+specials, 10% presorted runs).  Element g is at local position
+ * g % n of array g / n; records get payload g (bench.cu:189).              */
+int cs_generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, void* out,
+                hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUSORT_HIP_H */
