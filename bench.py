@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_i32",
-                    choices=["batched_i32", "batched_rec32"])
+                    choices=["batched_i32", "batched_rec32", "copy"])
+    ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--arrays", type=int, default=4096)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--method", default="auto")
@@ -59,38 +60,36 @@ def load_traffic(name: str):
 
 
 def bench_batched(args, dtype_name: str):
-    import torch
     import cusort_hip as cs
+    import hiprt
 
     dt = cs.DTYPE_NAMES[dtype_name]
     method = cs.METHODS.index(args.method)
     n, arrays = args.n, args.arrays
     count = n * arrays
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    stream = torch.cuda.Stream(device=dev)       # default priority
+    hiprt.set_device(0)
+    stream = hiprt.Stream()                      # default priority
     src = cs.device_buffer(dt, count)
     dst = cs.device_buffer(dt, count)
     seed = 42
-    with torch.cuda.stream(stream):
-        cs.generate(dt, cs.D_UNIFORM, n, count, seed, src.data_ptr(), stream.cuda_stream)
+    cs.generate(dt, cs.D_UNIFORM, n, count, seed, src.ptr, stream.handle)
     stream.synchronize()
 
     def step():
-        cs.batched_sort(dt, src.data_ptr(), dst.data_ptr(), arrays, n, method, args.threads, 0,
-                        stream.cuda_stream)
+        cs.batched_sort(dt, src.ptr, dst.ptr, arrays, n, method, args.threads, 0, stream.handle)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
+    hiprt.synchronize()
+    # HIP events on the stream the kernels run on bracket exactly K launches
+    start, end = hiprt.Event(), hiprt.Event()
     start.record(stream)
     for _ in range(args.steps):
         step()
     end.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
+    end.synchronize()
+    hiprt.synchronize()
+    ms = start.elapsed_ms(end) / args.steps
 
     # correctness of the measured output (outside the timed region)
     res = cs.to_numpy(dst, dt, count)
@@ -134,6 +133,34 @@ def bench_batched(args, dtype_name: str):
     return line
 
 
+def bench_copy(args):
+    """Plain device copy (cs_device_copy) of --copy-mib MiB: the measured HBM
+    copy bandwidth, and the known-traffic kernel that calibrates the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import cusort_hip as cs
+    import hiprt
+
+    hiprt.set_device(0)
+    stream = hiprt.Stream()
+    nbytes = args.copy_mib << 20
+    src, dst = hiprt.Buffer(nbytes), hiprt.Buffer(nbytes)
+    src.fill(1)
+    for _ in range(args.warmup):
+        cs.device_copy(dst.ptr, src.ptr, nbytes, stream.handle)
+    hiprt.synchronize()
+    start, end = hiprt.Event(), hiprt.Event()
+    start.record(stream)
+    for _ in range(args.steps):
+        cs.device_copy(dst.ptr, src.ptr, nbytes, stream.handle)
+    end.record(stream)
+    end.synchronize()
+    ms = start.elapsed_ms(end) / args.steps
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    return {"metric": "device copy bandwidth (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "bytes_per_launch": 2 * nbytes, "config": {"workload": f"copy {args.copy_mib} MiB"}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -143,6 +170,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched_i32":
         line = bench_batched(args, "i32")
+    elif args.workload == "copy":
+        line = bench_copy(args)
     else:
         line = bench_batched(args, "rec32")
     print(json.dumps(line), flush=True)

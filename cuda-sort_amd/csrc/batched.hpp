@@ -29,10 +29,15 @@ constexpr int max_logw(int dtype) {
            : (dtype == CS_REC_I64_I32) ? kMaxLogW16 : kMaxLogW8;
 }
 
-// Default elements per lane of the register tile for W = 2^logw.
+// Default elements per lane of the register tile for W = 2^logw: 32 VGPRs
+// of keys (16 for 16-byte records).  With E = 32 a wave holds two arrays of
+// 1024, an array needs sum_{k>5} ceil(k/5) LDS transposes (10 at W = 1024,
+// against 14 at E = 16), and every transpose of W <= 2048 is free of LDS
+// bank conflicts under the x + (x >> 5) padding (32 banks served per
+// 32-lane half of the wave).
 constexpr int auto_loge(int dtype, int logw) {
-    (void)dtype;
-    return logw < 4 ? logw : 4;
+    const int cap = dtype == CS_REC_I64_I32 ? 4 : 5;
+    return logw < cap ? logw : cap;
 }
 
 template <class E>
@@ -53,10 +58,10 @@ constexpr bool tile_exists(int logw, int loge) {
     if (logw > 0 && loge == 0) return false;                    // needs at least one slot bit
     if (logw - loge > 10) return false;                        // at most 1024 lanes
     if (tile_lds_bytes<Elem<DT>>(logw, loge) > 160 * 1024) return false;
-    // The reference's own types get every v6 tiling (items 1..16); the others
-    // only the tuned default.
-    if (DT == CS_I32 || DT == CS_REC_I32_I32) return loge <= 4;
-    return loge == auto_loge(DT, logw);
+    // The default tiling, plus every v6 tiling (items 1..16) for the
+    // reference's own element types.
+    if (loge == auto_loge(DT, logw)) return true;
+    return (DT == CS_I32 || DT == CS_REC_I32_I32) && loge <= 4;
 }
 
 template <int DT, int LW, int LE>
@@ -77,11 +82,14 @@ hipError_t launch_lds(const void* in, void* out, int64_t arrays, int n, int logw
     using S = typename E::Storage;
     if (arrays <= 0) return hipSuccess;
     const size_t bytes = (size_t(1) << logw) * (sizeof(typename E::Key) + (E::kPayload ? 4 : 0));
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bitonic_lds<E>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
+    if (bytes > 64 * 1024) {
+        // Dynamic LDS beyond 64 KB must be opted into per kernel.
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bitonic_lds<E>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();   // do not leave the error pending for the next call
+            return e;
+        }
     }
     hipLaunchKernelGGL(k_bitonic_lds<E>, dim3(unsigned(arrays)), dim3(unsigned(threads)), bytes, s,
                        static_cast<const S*>(in), static_cast<S*>(out), arrays, n, logw, flags);
@@ -96,10 +104,12 @@ inline void hfor(F&& f) {
     }
 }
 
-template <int DT>
+// Tile kernels of `DT` with LO <= log2(W) < HI (one translation unit each).
+template <int DT, int LO, int HI>
 KernelInfo tile_lookup(int logw, int loge) {
     KernelInfo r{};
-    hfor<0, max_logw(DT) + 1>([&](auto w_) {
+    constexpr int hi = HI < max_logw(DT) + 1 ? HI : max_logw(DT) + 1;
+    hfor<LO, hi>([&](auto w_) {
         constexpr int LW = decltype(w_)::value;
         hfor<0, 6>([&](auto e_) {
             constexpr int LE = decltype(e_)::value;
@@ -132,13 +142,19 @@ KernelInfo lds_lookup() {
 KernelInfo tile_kernel(int dtype, int logw, int loge);
 KernelInfo lds_kernel(int dtype);
 
-KernelInfo tile_kernel_i32(int, int);
-KernelInfo tile_kernel_u32(int, int);
-KernelInfo tile_kernel_f32(int, int);
-KernelInfo tile_kernel_u64(int, int);
-KernelInfo tile_kernel_i64(int, int);
-KernelInfo tile_kernel_rec32(int, int);
-KernelInfo tile_kernel_rec64(int, int);
+// Split point of the two translation units per element type.
+constexpr int kSplitLogW = 10;
+#define CS_DECLARE_TILES(name)                  \
+    KernelInfo tile_kernel_##name##_lo(int, int); \
+    KernelInfo tile_kernel_##name##_hi(int, int);
+CS_DECLARE_TILES(i32)
+CS_DECLARE_TILES(u32)
+CS_DECLARE_TILES(f32)
+CS_DECLARE_TILES(u64)
+CS_DECLARE_TILES(i64)
+CS_DECLARE_TILES(rec32)
+CS_DECLARE_TILES(rec64)
+#undef CS_DECLARE_TILES
 KernelInfo lds_kernel_i32();
 KernelInfo lds_kernel_u32();
 KernelInfo lds_kernel_f32();

@@ -1,0 +1,6 @@
+// Batched-sort tile kernels for CS_REC_I64_I32 with W >= 2^kSplitLogW (see batched.hpp).
+#include "batched.hpp"
+
+namespace cs {
+KernelInfo tile_kernel_rec64_hi(int logw, int loge) { return tile_lookup<CS_REC_I64_I32, kSplitLogW, 32>(logw, loge); }
+}  // namespace cs

@@ -160,14 +160,59 @@ struct Tile {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
+
+    // Lane-striped I/O order: slot r of lane t is array position
+    // ((r / VS) * T + t) * VS + r % VS, so for every group of VS slots the
+    // lanes of an array cover consecutive 16-byte chunks (coalesced global
+    // accesses).  io_transpose moves the tile between that order and the
+    // identity layout L(e-1) (slot r of lane t = position t*NE + r) through
+    // LDS; both sides are additive lane/slot address splits.
+    template <int VS, bool kToIdentity>
+    __device__ __forceinline__ void io_transpose(int t, Key* lk, int32_t* lp, int buf) {
+        constexpr int PS = C::PADSHIFT;
+        const int sa = lds_addr(t * VS, PS) + buf;
+        const int ia = lds_addr(t * NE, PS) + buf;
+        sfor<0, NE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            constexpr int soff = lds_addr((r / VS) * C::T * VS + (r % VS), PS);
+            constexpr int ioff = lds_addr(r, PS);
+            const int a = kToIdentity ? sa + soff : ia + ioff;
+            lk[a] = k[r];
+            if constexpr (E::kPayload) lp[a] = p[r];
+        });
+        if constexpr (C::kMultiWave) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        sfor<0, NE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            constexpr int soff = lds_addr((r / VS) * C::T * VS + (r % VS), PS);
+            constexpr int ioff = lds_addr(r, PS);
+            const int a = kToIdentity ? ia + ioff : sa + soff;
+            k[r] = lk[a];
+            if constexpr (E::kPayload) p[r] = lp[a];
+        });
+        if constexpr (!C::kMultiWave) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
 };
 
 // ------------------------------------------------------------ the kernel --
-// Global I/O.  Keys without payload may be loaded in any order (the sorted
-// output does not depend on it, and max-key padding cannot enter the valid
-// prefix of a key-only sort), so they are read with lanes striding over
-// consecutive 16-byte chunks.  Records are loaded position for position
-// (logical = input index) so the padding argument above applies.
+// Global I/O is lane-striped (coalesced 16-byte accesses).  Keys without
+// payload are loaded straight into the registers in that order and the
+// order is simply declared to be the identity layout: the sorted output
+// does not depend on the input order, and max-key padding cannot enter the
+// valid prefix of a key-only sort.  Records must start with logical
+// position = input position (so the padding guarantee applies), so their
+// striped load is re-laid to the identity layout by one LDS transpose.  The
+// sorted tile leaves the identity layout through one more LDS transpose so
+// that the stores are lane-striped too.
 template <class E, int LOGW, int LOGE>
 __global__ void __launch_bounds__((TileCfg<E, LOGW, LOGE>::BLOCK))
 k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out,
@@ -178,6 +223,8 @@ k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out,
     constexpr int NE = C::NE;
     constexpr int T = C::T;
     constexpr int V = 16 / int(sizeof(Storage)) > 0 ? 16 / int(sizeof(Storage)) : 1;   // elements per 16 B
+    constexpr int VS = V < NE ? V : NE;                                               // per striped group
+    using Vec = __attribute__((ext_vector_type(4))) uint32_t;
 
     __shared__ Key lds_k[C::LDS_KEYS];
     __shared__ int32_t lds_p[C::LDS_PAYLOAD];
@@ -190,91 +237,69 @@ k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out,
     const Storage* src = in + (active ? a : 0) * int64_t(n);
     Storage* dst = out + (active ? a : 0) * int64_t(n);
     const int valid = active ? n : 0;
+    Key* lk = lds_k + (C::NTRANS ? group * C::REGION : 0);
+    int32_t* lp = lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0);
+    constexpr int kBufLast = C::kMultiWave ? (C::NTRANS & 1) * C::APB * C::REGION : 0;
+    constexpr int kBufFirst = C::kMultiWave ? C::APB * C::REGION : 0;
 
     Tile<E, LOGW, LOGE> tile;
     // 16-byte accesses need 16-byte aligned array starts (n*sizeof % 16 == 0).
-    const bool vec_in = ((uintptr_t)src & 15) == 0 && V > 1 && NE >= V;
-    const bool vec_out = ((uintptr_t)dst & 15) == 0 && V > 1 && NE >= V;
+    const bool vec_in = ((uintptr_t)src & 15) == 0 && VS == V && V > 1;
+    const bool vec_out = ((uintptr_t)dst & 15) == 0 && VS == V && V > 1;
 
-    // ---- load
-    if constexpr (!E::kPayload) {
-        if (vec_in) {
-            sfor<0, NE / (V > 1 ? V : 1)>([&](auto q_) {
-                constexpr int q = decltype(q_)::value;
-                const int g = (q * T + t) * V;
-                if (g + V <= valid) {
-                    using Vec = __attribute__((ext_vector_type(4))) uint32_t;
-                    const Vec v = *reinterpret_cast<const Vec*>(src + g);
-                    Storage tmp[V];
-                    __builtin_memcpy(tmp, &v, 16);
-                    sfor<0, V>([&](auto j_) {
-                        constexpr int j = decltype(j_)::value;
-                        tile.k[q * V + j] = E::key(tmp[j]);
-                    });
-                } else {
-                    sfor<0, V>([&](auto j_) {
-                        constexpr int j = decltype(j_)::value;
-                        tile.k[q * V + j] = (g + j < valid) ? E::key(src[g + j]) : E::kPad;
-                    });
-                }
+    // ---- load, lane-striped
+    sfor<0, NE / VS>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        const int g = (q * T + t) * VS;
+        if (vec_in && g + VS <= valid) {
+            const Vec v = *reinterpret_cast<const Vec*>(src + g);
+            Storage tmp[V];
+            __builtin_memcpy(tmp, &v, 16);
+            sfor<0, VS>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                tile.k[q * VS + j] = E::key(tmp[j]);
+                tile.p[q * VS + j] = E::payload(tmp[j]);
             });
         } else {
-            sfor<0, NE>([&](auto r_) {
-                constexpr int r = decltype(r_)::value;
-                const int g = r * T + t;
-                tile.k[r] = g < valid ? E::key(src[g]) : E::kPad;
+            sfor<0, VS>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                if (g + j < valid) {
+                    const Storage s = src[g + j];
+                    tile.k[q * VS + j] = E::key(s);
+                    tile.p[q * VS + j] = E::payload(s);
+                } else {
+                    tile.k[q * VS + j] = E::kPad;
+                    tile.p[q * VS + j] = INT32_MAX;
+                }
             });
         }
-        sfor<0, NE>([&](auto r_) { tile.p[decltype(r_)::value] = 0; });
-    } else {
-        sfor<0, NE>([&](auto r_) {
-            constexpr int r = decltype(r_)::value;
-            const int g = t * NE + r;
-            if (g < valid) {
-                const Storage s = src[g];
-                tile.k[r] = E::key(s);
-                tile.p[r] = E::payload(s);
-            } else {
-                tile.k[r] = E::kPad;
-                tile.p[r] = INT32_MAX;
-            }
-        });
-    }
+    });
+    if constexpr (E::kPayload && C::NTRANS > 0) tile.template io_transpose<VS, true>(t, lk, lp, kBufFirst);
 
     // ---- network
-    Key* lk = lds_k + (C::NTRANS ? group * C::REGION : 0);
-    int32_t* lp = lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0);
     tile.template run<0>(t, lk, lp);
 
-    // ---- store: the final layout is L(e-1), lane t holds t*NE .. t*NE+NE-1
-    if (vec_out && !E::kPayload) {
-        sfor<0, NE / (V > 1 ? V : 1)>([&](auto q_) {
-            constexpr int q = decltype(q_)::value;
-            const int g = t * NE + q * V;
-            if (g + V <= valid) {
-                Storage tmp[V];
-                sfor<0, V>([&](auto j_) {
-                    constexpr int j = decltype(j_)::value;
-                    tmp[j] = E::make(tile.k[q * V + j], 0);
-                });
-                using Vec = __attribute__((ext_vector_type(4))) uint32_t;
-                Vec v;
-                __builtin_memcpy(&v, tmp, 16);
-                *reinterpret_cast<Vec*>(dst + g) = v;
-            } else {
-                sfor<0, V>([&](auto j_) {
-                    constexpr int j = decltype(j_)::value;
-                    if (g + j < valid) dst[g + j] = E::make(tile.k[q * V + j], 0);
-                });
-            }
-        });
-    } else {
-        sfor<0, NE>([&](auto r_) {
-            constexpr int r = decltype(r_)::value;
-            const int g = t * NE + r;
-            if (g < valid) dst[g] = E::make(tile.k[r], tile.p[r]);
-        });
-    }
+    // ---- store, lane-striped
+    if constexpr (C::NTRANS > 0) tile.template io_transpose<VS, false>(t, lk, lp, kBufLast);
+    sfor<0, NE / VS>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        const int g = (q * T + t) * VS;
+        if (vec_out && g + VS <= valid) {
+            Storage tmp[V];
+            sfor<0, VS>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                tmp[j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
+            });
+            Vec v;
+            __builtin_memcpy(&v, tmp, 16);
+            *reinterpret_cast<Vec*>(dst + g) = v;
+        } else {
+            sfor<0, VS>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                if (g + j < valid) dst[g + j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
+            });
+        }
+    });
 }
 
 }  // namespace cs

@@ -11,16 +11,20 @@
 namespace cs {
 
 KernelInfo tile_kernel(int dtype, int logw, int loge) {
+    const bool hi = logw >= kSplitLogW;
+#define CS_TILE_CASE(code, name) \
+    case code: return hi ? tile_kernel_##name##_hi(logw, loge) : tile_kernel_##name##_lo(logw, loge);
     switch (dtype) {
-        case CS_I32: return tile_kernel_i32(logw, loge);
-        case CS_U32: return tile_kernel_u32(logw, loge);
-        case CS_F32: return tile_kernel_f32(logw, loge);
-        case CS_U64: return tile_kernel_u64(logw, loge);
-        case CS_I64: return tile_kernel_i64(logw, loge);
-        case CS_REC_I32_I32: return tile_kernel_rec32(logw, loge);
-        case CS_REC_I64_I32: return tile_kernel_rec64(logw, loge);
+        CS_TILE_CASE(CS_I32, i32)
+        CS_TILE_CASE(CS_U32, u32)
+        CS_TILE_CASE(CS_F32, f32)
+        CS_TILE_CASE(CS_U64, u64)
+        CS_TILE_CASE(CS_I64, i64)
+        CS_TILE_CASE(CS_REC_I32_I32, rec32)
+        CS_TILE_CASE(CS_REC_I64_I32, rec64)
         default: return KernelInfo{};
     }
+#undef CS_TILE_CASE
 }
 
 KernelInfo lds_kernel(int dtype) {
@@ -222,6 +226,14 @@ int cs_batched_describe(int dtype, int32_t n, int method, int32_t threads, uint3
         else *vgprs = -1;
     }
     return CS_OK;
+}
+
+int cs_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) return CS_OK;
+    if (!dst || !src) return CS_E_INVALID_VALUE;
+    if (overlaps_partially(src, dst, bytes) || src == dst) return src == dst ? CS_OK : CS_E_INVALID_VALUE;
+    const hipError_t e = cs::device_copy(dst, src, bytes, stream);
+    return e == hipSuccess ? CS_OK : int(e);
 }
 
 int cs_generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, void* out, hipStream_t stream) {

@@ -132,7 +132,35 @@ __global__ void k_generate(int dtype, int dist, int64_t n, int64_t count, uint64
     }
 }
 
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_copy1(const unsigned char* __restrict__ src, unsigned char* __restrict__ dst,
+                                             size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    if ((((uintptr_t)dst | (uintptr_t)src | bytes) & 15) == 0) {
+        const size_t n16 = bytes / 16;
+        const size_t want = (n16 + 255) / 256;
+        const unsigned blocks = unsigned(want < 8192 ? want : 8192);
+        hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, static_cast<const uint4*>(src),
+                           static_cast<uint4*>(dst), n16);
+    } else {
+        const size_t want = (bytes + 255) / 256;
+        const unsigned blocks = unsigned(want < 8192 ? want : 8192);
+        hipLaunchKernelGGL(k_copy1, dim3(blocks), dim3(256), 0, s, static_cast<const unsigned char*>(src),
+                           static_cast<unsigned char*>(dst), bytes);
+    }
+    return hipGetLastError();
+}
 
 hipError_t generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, void* out, hipStream_t s) {
     if (count <= 0) return hipSuccess;

@@ -1,8 +1,8 @@
 """Python binding of libcusort_hip.so (the C ABI in include/cusort_hip.h).
 
 The product is the HIP library; this module is the thin host-side mirror the
-tests and the benchmark call through.  PyTorch, when used, only provides
-device memory and streams.  There is no CPU fallback: if the library is
+tests and the benchmark call through.  Device memory, streams and events come
+from the HIP runtime the library links (hiprt.py).  There is no CPU fallback: if the library is
 missing or no GPU is present the calls fail loudly.
 """
 from __future__ import annotations
@@ -73,6 +73,8 @@ def lib():
         L.cs_batched_max_n.argtypes = [ctypes.c_int]
         L.cs_batched_describe.restype = ctypes.c_int
         L.cs_batched_describe.argtypes = [ctypes.c_int, i32, ctypes.c_int, i32, u32, P32, P32, P32, P32, P32]
+        L.cs_device_copy.restype = ctypes.c_int
+        L.cs_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.cs_generate.restype = ctypes.c_int
         L.cs_generate.argtypes = [ctypes.c_int, ctypes.c_int, i64, i64, u64, vp, vp]
         _lib = L
@@ -124,27 +126,32 @@ def generate(dtype: int, dist: int, n: int, count: int, seed: int, dst: int, str
     check(lib().cs_generate(dtype, dist, n, count, seed & (2**64 - 1), dst, stream), "cs_generate")
 
 
-# ------------------------------------------------------------ torch helpers
-def device_buffer(dtype: int, count: int, device: str = "cuda"):
-    """Raw byte buffer for `count` elements of `dtype` on the GPU (torch)."""
-    import torch
-    return torch.empty(max(1, count * DTYPE_SIZE[dtype]), dtype=torch.uint8, device=device)
+def device_copy(dst: int, src: int, nbytes: int, stream: int = 0) -> None:
+    check(lib().cs_device_copy(dst, src, nbytes, stream), "cs_device_copy")
 
 
-def to_numpy(buf, dtype: int, count: int):
-    """Copy `count` elements of a device byte buffer to a numpy array."""
+# ------------------------------------------------------------ buffers
+def _np_dtype(dtype: int):
     import numpy as np
-    np_dtype = {I32: np.dtype("<i4"), F32: np.dtype("<u4"), U32: np.dtype("<u4"), U64: np.dtype("<u8"),
-                I64: np.dtype("<i8"),
-                REC_I32_I32: np.dtype([("key", "<i4"), ("payload", "<i4")]),
-                REC_I64_I32: np.dtype([("key", "<i8"), ("payload", "<i4"), ("pad", "<i4")])}[dtype]
-    raw = buf[: count * DTYPE_SIZE[dtype]].cpu().numpy()
-    return raw.view(np_dtype).copy()
+    return {I32: np.dtype("<i4"), F32: np.dtype("<u4"), U32: np.dtype("<u4"), U64: np.dtype("<u8"),
+            I64: np.dtype("<i8"),
+            REC_I32_I32: np.dtype([("key", "<i4"), ("payload", "<i4")]),
+            REC_I64_I32: np.dtype([("key", "<i8"), ("payload", "<i4"), ("pad", "<i4")])}[dtype]
 
 
-def from_numpy(arr, device: str = "cuda"):
-    """Upload a numpy array as a device byte buffer."""
-    import numpy as np
-    import torch
-    raw = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
-    return torch.from_numpy(raw.copy()).to(device)
+def device_buffer(dtype: int, count: int):
+    """Device allocation for `count` elements of `dtype` (hiprt.Buffer)."""
+    import hiprt
+    return hiprt.Buffer(count * DTYPE_SIZE[dtype])
+
+
+def to_numpy(buf, dtype: int, count: int, offset_elems: int = 0):
+    """Copy `count` elements starting at element `offset_elems` to the host."""
+    return buf.download(_np_dtype(dtype), count, offset_elems * DTYPE_SIZE[dtype])
+
+
+def from_numpy(arr):
+    import hiprt
+    b = hiprt.Buffer(arr.nbytes)
+    b.upload(arr)
+    return b

@@ -129,6 +129,13 @@ int cs_batched_describe(int dtype, int32_t n, int method, int32_t threads, uint3
                         int32_t* block_threads, int32_t* items_per_thread, int32_t* lds_bytes,
                         int32_t* vgprs, int32_t* arrays_per_block);
 
+/* Device-to-device copy of `bytes` bytes by a plain streaming kernel (16-byte
+ * accesses when both pointers are 16-byte aligned).  The reference harness
+ * restores every input with a device copy before each trial
+ * (bench.cu:304-307, 335); the benchmark also uses this kernel, whose HBM
+ * traffic is known exactly, to calibrate rocprofv3's FETCH_SIZE/WRITE_SIZE. */
+int cs_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 /* Input distributions of cs_generate (numbering shared with the oracle). */
 enum cs_dist {
     CS_DIST_UNIFORM = 0,        /* uniform over the full key range            */

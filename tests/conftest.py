@@ -30,7 +30,7 @@ def cs():
 
 @pytest.fixture(scope="session")
 def gpu(cs):
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a HIP device"
-    torch.cuda.init()
-    return torch
+    import hiprt
+    assert hiprt.device_count() > 0, "GPU tests need a HIP device"
+    hiprt.set_device(0)
+    return hiprt

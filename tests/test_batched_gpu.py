@@ -12,31 +12,31 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def run_case(cs, orc, torch, dtype, dist, n, arrays, method=0, threads=0, flags=0, inplace=False,
+def run_case(cs, orc, rt, dtype, dist, n, arrays, method=0, threads=0, flags=0, inplace=False,
              seed=42, offset_elems=0):
     count = n * arrays
     esz = cs.DTYPE_SIZE[dtype]
     base = cs.device_buffer(dtype, count + offset_elems + 1)
-    src_ptr = base.data_ptr() + offset_elems * esz
+    src_ptr = base.ptr + offset_elems * esz
     cs.generate(dtype, dist, n, count, seed, src_ptr)
     host_in = orc.generate(dtype, dist, n, count, seed)
-    torch.cuda.synchronize()
-    dev_in = cs.to_numpy(base[offset_elems * esz:], dtype, count)
+    rt.synchronize()
+    dev_in = cs.to_numpy(base, dtype, count, offset_elems)
     assert dev_in.tobytes() == host_in.tobytes(), "device generator differs from the oracle's"
     if inplace:
         out, dst_ptr = base, src_ptr
     else:
         out = cs.device_buffer(dtype, count + offset_elems + 1)
-        dst_ptr = out.data_ptr() + offset_elems * esz
+        dst_ptr = out.ptr + offset_elems * esz
     cs.batched_sort(dtype, src_ptr, dst_ptr, arrays, n, method, threads, flags)
-    torch.cuda.synchronize()
-    res = cs.to_numpy(out[offset_elems * esz:], dtype, count)
+    rt.synchronize()
+    res = cs.to_numpy(out, dtype, count, offset_elems)
     rc, msg = orc.validate_batched(dtype, host_in, res, n)
     where = (f"dtype={dtype} dist={orc.DIST_NAMES[dist]} n={n} arrays={arrays} method={cs.METHODS[method]} "
              f"threads={threads} flags={flags} inplace={inplace}")
     assert rc == 0, f"{where}: {msg}"
     if not inplace:
-        again = cs.to_numpy(base[offset_elems * esz:], dtype, count)
+        again = cs.to_numpy(base, dtype, count, offset_elems)
         assert again.tobytes() == host_in.tobytes(), f"{where}: input modified by an out-of-place sort"
 
 
@@ -136,14 +136,14 @@ def test_many_tiny_arrays(cs, orc, gpu):
 
 def test_empty_is_noop(cs, gpu):
     buf = cs.device_buffer(cs.I32, 16)
-    cs.batched_sort(cs.I32, buf.data_ptr(), buf.data_ptr(), 0, 16)
-    cs.batched_sort(cs.I32, buf.data_ptr(), buf.data_ptr(), 4, 0)
-    gpu.cuda.synchronize()
+    cs.batched_sort(cs.I32, buf.ptr, buf.ptr, 0, 16)
+    cs.batched_sort(cs.I32, buf.ptr, buf.ptr, 4, 0)
+    gpu.synchronize()
 
 
 def test_describe_reports_resources(cs, gpu):
     d = cs.describe(cs.I32, 1024)
-    assert d["items_per_thread"] == 16 and d["block_threads"] == 256 and d["arrays_per_block"] == 4
+    assert d["items_per_thread"] == 32 and d["block_threads"] == 256 and d["arrays_per_block"] == 8
     assert 0 < d["vgprs"] <= 128
     d = cs.describe(cs.I32, 1000, cs.V3, 96)
     assert d["block_threads"] == 96 and d["arrays_per_block"] == 1
