@@ -37,7 +37,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_i32",
-                    choices=["batched_i32", "batched_rec32", "copy"])
+                    choices=["batched_i32", "batched_rec32", "copy", "ragged_i32", "ragged_f32"])
+    ap.add_argument("--segments", type=int, default=100000)
+    ap.add_argument("--max-len", type=int, default=8192)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--arrays", type=int, default=4096)
     ap.add_argument("--n", type=int, default=1024)
@@ -161,6 +164,75 @@ def bench_copy(args):
             "bytes_per_launch": 2 * nbytes, "config": {"workload": f"copy {args.copy_mib} MiB"}}
 
 
+def bench_ragged(args, dtype_name: str):
+    """Ragged batch: --segments arrays, lengths log-uniform over 1..--max-len
+    (seed 42), CSR offsets; persistent bucketed graph launch vs the plain
+    per-array launch (one lane group per segment, longest-segment tiling)."""
+    import numpy as np
+    import cusort_hip as cs
+    import hiprt
+
+    dt = cs.DTYPE_NAMES[dtype_name]
+    hiprt.set_device(0)
+    stream = hiprt.Stream()
+    lens = cs.ragged_lengths(args.segments, args.max_len, 42)
+    off = cs.offsets_from_lengths(lens)
+    total = int(off[-1])
+    dist = cs.D_UNIFORM if dt != cs.F32 else cs.D_NORMAL_SPECIAL
+    src, dst = cs.device_buffer(dt, total), cs.device_buffer(dt, total)
+    doff = cs.from_numpy(off)
+    cs.generate(dt, dist, total, total, 42, src.ptr, stream.handle)
+    if not args.no_tune:
+        cs.segmented_tune(dt, 3, stream.handle)
+    plan = cs.SegPlan(dt, off)
+    stream.synchronize()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        hiprt.synchronize()
+        start, end = hiprt.Event(), hiprt.Event()
+        start.record(stream)
+        for _ in range(args.steps):
+            fn()
+        end.record(stream)
+        end.synchronize()
+        return start.elapsed_ms(end) / args.steps
+
+    ms_p = timed(lambda: plan.sort(src.ptr, dst.ptr, doff.ptr, stream.handle))
+    res_p = cs.to_numpy(dst, dt, total)
+    ms_b = timed(lambda: cs.segmented_sort_plain(dt, src.ptr, dst.ptr, doff.ptr, args.segments, args.max_len,
+                                                 stream.handle))
+    res_b = cs.to_numpy(dst, dt, total)
+    same = res_p.tobytes() == res_b.tobytes()
+    ok, cpu = None, None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        host_in = oracle.generate(dt, dist, total, total, 42)
+        ok = oracle.validate_segmented(dt, host_in, res_p, off)[0] == 0
+        secs, threads = oracle.time_segmented(dt, host_in, off, threads=0, trials=3)
+        cpu = {"value": total / secs / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
+               "sample": f"the full ragged batch ({total} keys), per-segment std::sort over {threads} host "
+                         "threads balanced by n log n, median of 3 trials"}
+    bytes_per_key = cs.DTYPE_SIZE[dt] * 2
+    alg = total * bytes_per_key + off.nbytes
+    info = plan.info()
+    return {"metric": METRIC, "value": total / (ms_p * 1e-3) / 1e9, "unit": "Gkeys/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_p, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32" if dt == cs.I32 else "fp32 (totalOrder)",
+            "data": "synthetic",
+            "config": {"workload": f"ragged batch, {args.segments} segments, lengths log-uniform 1..{args.max_len} "
+                                   f"(seed 42), {total} {dtype_name} keys, persistent bucketed hipGraph launch",
+                       "segments": args.segments, "total_keys": total, "work_items": info["work_items"],
+                       "grid": info["grid"], "variants": info["variants"][:14]},
+            "plain_per_array": {"ms_per_step": ms_b, "value": total / (ms_b * 1e-3) / 1e9,
+                                "speedup_persistent_vs_plain": ms_b / ms_p, "identical_output": same},
+            "roofline": {"bound": "hbm", "achieved": alg / (ms_p * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+            "cpu_baseline": cpu, "validated": ok}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -172,6 +244,8 @@ def main():
         line = bench_batched(args, "i32")
     elif args.workload == "copy":
         line = bench_copy(args)
+    elif args.workload.startswith("ragged"):
+        line = bench_ragged(args, args.workload.split("_")[1])
     else:
         line = bench_batched(args, "rec32")
     print(json.dumps(line), flush=True)

@@ -213,32 +213,20 @@ struct Tile {
 // striped load is re-laid to the identity layout by one LDS transpose.  The
 // sorted tile leaves the identity layout through one more LDS transpose so
 // that the stores are lane-striped too.
+// Sorts one array of `valid` elements (valid <= W; 0 for an idle group)
+// from src into dst with the T lanes t = 0..T-1 of this group; lk / lp are
+// the group's LDS region (C::REGION words per buffer).  Every lane of the
+// workgroup must call it when C::kMultiWave (it contains barriers).
 template <class E, int LOGW, int LOGE>
-__global__ void __launch_bounds__((TileCfg<E, LOGW, LOGE>::BLOCK))
-k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out,
-               int64_t arrays, int n) {
+__device__ __forceinline__ void tile_sort(const typename E::Storage* src, typename E::Storage* dst, int valid,
+                                          int t, typename E::Key* lk, int32_t* lp) {
     using C = TileCfg<E, LOGW, LOGE>;
     using Storage = typename E::Storage;
-    using Key = typename E::Key;
     constexpr int NE = C::NE;
     constexpr int T = C::T;
     constexpr int V = 16 / int(sizeof(Storage)) > 0 ? 16 / int(sizeof(Storage)) : 1;   // elements per 16 B
     constexpr int VS = V < NE ? V : NE;                                               // per striped group
     using Vec = __attribute__((ext_vector_type(4))) uint32_t;
-
-    __shared__ Key lds_k[C::LDS_KEYS];
-    __shared__ int32_t lds_p[C::LDS_PAYLOAD];
-
-    const int tid = int(threadIdx.x);
-    const int group = tid >> C::LOGT;
-    const int t = tid & (T - 1);
-    const int64_t a = int64_t(blockIdx.x) * C::APB + group;
-    const bool active = a < arrays;
-    const Storage* src = in + (active ? a : 0) * int64_t(n);
-    Storage* dst = out + (active ? a : 0) * int64_t(n);
-    const int valid = active ? n : 0;
-    Key* lk = lds_k + (C::NTRANS ? group * C::REGION : 0);
-    int32_t* lp = lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0);
     constexpr int kBufLast = C::kMultiWave ? (C::NTRANS & 1) * C::APB * C::REGION : 0;
     constexpr int kBufFirst = C::kMultiWave ? C::APB * C::REGION : 0;
 
@@ -300,6 +288,23 @@ k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out,
             });
         }
     });
+}
+
+// Batched launch: array a = in + a*n, APB arrays per workgroup.
+template <class E, int LOGW, int LOGE>
+__global__ void __launch_bounds__((TileCfg<E, LOGW, LOGE>::BLOCK))
+k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out, int64_t arrays, int n) {
+    using C = TileCfg<E, LOGW, LOGE>;
+    __shared__ typename E::Key lds_k[C::LDS_KEYS];
+    __shared__ int32_t lds_p[C::LDS_PAYLOAD];
+    const int tid = int(threadIdx.x);
+    const int group = tid >> C::LOGT;
+    const int t = tid & (C::T - 1);
+    const int64_t a = int64_t(blockIdx.x) * C::APB + group;
+    const bool active = a < arrays;
+    tile_sort<E, LOGW, LOGE>(in + (active ? a : 0) * int64_t(n), out + (active ? a : 0) * int64_t(n),
+                             active ? n : 0, t, lds_k + (C::NTRANS ? group * C::REGION : 0),
+                             lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
 }
 
 }  // namespace cs

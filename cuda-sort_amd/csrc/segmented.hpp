@@ -1,0 +1,274 @@
+// Ragged batches: many arrays of uneven length given as CSR offsets
+// (segment s = [off[s], off[s+1])).
+//
+// Persistent bucketed kernel.  At plan time every segment is put in the
+// bucket b = ceil(log2(len)) (W = 2^b) and the segments of a bucket are cut
+// into work items of as many segments as one workgroup of 256 threads sorts
+// at once with that bucket's tiling.  Items are listed largest bucket
+// first.  One launch of k_seg_persistent (a grid of a few workgroups per
+// CU) pulls items from a global counter until the list is exhausted; each
+// item runs the tile sort of its bucket with the variant the tuner chose:
+//   variant 0: register tile, 32 keys per lane (16 for 16-byte records)
+//   variant 1: register tile, 16 keys per lane
+//   variant 2: LDS-resident network, one segment per workgroup
+// Every wave reaches the loop exit once the counter passes the item count.
+//
+// The reference's only ragged sort is SortPartitions (cusort.hpp:1802-1846):
+// one block per bucket with LDS sized for the largest bucket.
+// k_seg_plain is that baseline on this hardware: one lane group per
+// segment, every segment padded to the maximum length's tiling.
+#pragma once
+
+#include "batched.hpp"
+
+namespace cs {
+
+constexpr int kSegBlock = 256;      // threads per workgroup of the segmented kernels
+constexpr int kSegVariants = 3;
+
+struct SegItem {
+    int bucket;    // log2 W
+    int variant;   // 0, 1, 2 (see above)
+    int first;     // index into the bucket-ordered segment list
+    int count;     // segments in this item
+};
+
+template <int DT> constexpr int seg_max_logw() {
+    return sizeof(typename Elem<DT>::Storage) == 4 ? 13 : (sizeof(typename Elem<DT>::Storage) == 8 ? 12 : 12);
+}
+
+// Tiling (log2 E) of a register-tile variant for bucket b; -1 if absent.
+template <int DT> constexpr int seg_loge(int b, int variant) {
+    const int cap = DT == CS_REC_I64_I32 ? 4 : 5;
+    if (variant == 0) return b < cap ? b : cap;
+    if (variant == 1) return b < 4 ? -1 : 4;     // distinct from variant 0 only when b >= 5
+    return -1;
+}
+
+template <int DT> constexpr bool seg_variant_ok(int b, int variant) {
+    if (b < 0 || b > seg_max_logw<DT>()) return false;
+    if (variant == 2) return b >= 1;
+    const int le = seg_loge<DT>(b, variant);
+    if (le < 0 || (variant == 1 && b < 5)) return false;
+    if (b > 0 && le == 0) return false;
+    return (1 << (b - le)) <= kSegBlock;
+}
+
+template <int DT> constexpr int seg_segments_per_item(int b, int variant) {
+    if (variant == 2) return 1;
+    const int T = 1 << (b - seg_loge<DT>(b, variant));
+    return kSegBlock / T;
+}
+
+template <class E> constexpr int tile_lds_words(int logw, int loge) {
+    const int T = 1 << (logw - loge);
+    const int apb = (T >= 256 ? T : 256) / T;
+    const int padshift = sizeof(typename E::Key) == 4 ? 5 : 4;
+    const int region = (1 << logw) + ((1 << logw) >> padshift);
+    return make_plan(logw, loge).ntrans ? (T > 64 ? 2 : 1) * apb * region : 0;
+}
+
+// LDS words (keys; payload uses the same count of int32) the persistent
+// kernel needs: the maximum over all bucket variants.
+template <int DT> constexpr int seg_lds_words() {
+    int m = 1 << seg_max_logw<DT>();    // LDS-resident variant: one segment of W
+    for (int b = 0; b <= seg_max_logw<DT>(); ++b)
+        for (int v = 0; v < 2; ++v)
+            if (seg_variant_ok<DT>(b, v)) {
+                const int w = tile_lds_words<Elem<DT>>(b, seg_loge<DT>(b, v));
+                m = w > m ? w : m;
+            }
+    return m;
+}
+
+// LDS-resident network for one segment with all threads of the block.
+template <class E>
+__device__ void lds_segment_sort(const typename E::Storage* src, typename E::Storage* dst, int n, int logw,
+                                 typename E::Key* lk, int32_t* lp) {
+    const int W = 1 << logw;
+    const int T = int(blockDim.x);
+    const int tid = int(threadIdx.x);
+    for (int i = tid; i < W; i += T) {
+        if (i < n) {
+            const typename E::Storage s = src[i];
+            lk[i] = E::key(s);
+            if constexpr (E::kPayload) lp[i] = E::payload(s);
+        } else {
+            lk[i] = E::kPad;
+            if constexpr (E::kPayload) lp[i] = INT32_MAX;
+        }
+    }
+    __syncthreads();
+    for (int k = 1; k <= logw; ++k) {
+        const int half = 1 << (k - 1);
+        for (int q = tid; q < W / 2; q += T) {
+            const int i = ((q >> (k - 1)) << k) | (q & (half - 1));
+            lds_cx<E>(lk, lp, i, i ^ ((half << 1) - 1));
+        }
+        __syncthreads();
+        for (int d = half >> 1; d >= 1; d >>= 1) {
+            const int ld = __builtin_ctz(d);
+            for (int q = tid; q < W / 2; q += T) {
+                const int i = ((q >> ld) << (ld + 1)) | (q & (d - 1));
+                lds_cx<E>(lk, lp, i, i + d);
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < n; i += T) dst[i] = E::make(lk[i], E::kPayload ? lp[i] : 0);
+    __syncthreads();
+}
+
+// One item: `count` segments of bucket B (list entries first..first+count-1)
+// with register-tile variant V.
+template <int DT, int B, int VAR>
+__device__ __forceinline__ void seg_tile_item(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* out,
+                                              const int64_t* off, const int* list, int first, int count,
+                                              typename Elem<DT>::Key* lk, int32_t* lp) {
+    using E = Elem<DT>;
+    constexpr int LE = seg_loge<DT>(B, VAR);
+    using C = TileCfg<E, B, LE>;
+    const int tid = int(threadIdx.x);
+    const int group = tid >> C::LOGT;
+    const int t = tid & (C::T - 1);
+    const bool active = group < count;
+    const int s = active ? list[first + group] : 0;
+    const int64_t b0 = active ? off[s] : 0;
+    const int len = active ? int(off[s + 1] - b0) : 0;
+    tile_sort<E, B, LE>(in + b0, out + b0, len, t, lk + (C::NTRANS ? group * C::REGION : 0),
+                        lp + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
+}
+
+template <int DT, int B>
+__device__ __forceinline__ void seg_bucket(int variant, const typename Elem<DT>::Storage* in,
+                                           typename Elem<DT>::Storage* out, const int64_t* off, const int* list,
+                                           int first, int count, typename Elem<DT>::Key* lk, int32_t* lp) {
+    if constexpr (seg_variant_ok<DT>(B, 0)) {
+        if (variant == 0) { seg_tile_item<DT, B, 0>(in, out, off, list, first, count, lk, lp); return; }
+    }
+    if constexpr (seg_variant_ok<DT>(B, 1)) {
+        if (variant == 1) { seg_tile_item<DT, B, 1>(in, out, off, list, first, count, lk, lp); return; }
+    }
+    if constexpr (seg_variant_ok<DT>(B, 2)) {
+        if (variant == 2) {
+            const int s = list[first];
+            const int64_t b0 = off[s];
+            lds_segment_sort<Elem<DT>>(in + b0, out + b0, int(off[s + 1] - b0), B, lk, lp);
+        }
+    }
+}
+
+template <int DT, int B = 0>
+__device__ __forceinline__ void seg_dispatch(int bucket, int variant, const typename Elem<DT>::Storage* in,
+                                             typename Elem<DT>::Storage* out, const int64_t* off, const int* list,
+                                             int first, int count, typename Elem<DT>::Key* lk, int32_t* lp) {
+    if constexpr (B <= seg_max_logw<DT>()) {
+        if (bucket == B) {
+            seg_bucket<DT, B>(variant, in, out, off, list, first, count, lk, lp);
+            return;
+        }
+        seg_dispatch<DT, B + 1>(bucket, variant, in, out, off, list, first, count, lk, lp);
+    }
+}
+
+template <int DT>
+__global__ void __launch_bounds__(kSegBlock)
+k_seg_persistent(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* out, const int64_t* off,
+                 const int* list, const SegItem* items, int nitems, int* counter) {
+    using E = Elem<DT>;
+    __shared__ typename E::Key lds_k[seg_lds_words<DT>()];
+    __shared__ int32_t lds_p[E::kPayload ? seg_lds_words<DT>() : 1];
+    __shared__ int item_slot;
+    for (;;) {
+        if (threadIdx.x == 0) item_slot = atomicAdd(counter, 1);
+        __syncthreads();
+        const int i = item_slot;
+        __syncthreads();                       // everyone has read the slot before it is reused
+        if (i >= nitems) break;                // uniform: every wave leaves together
+        const SegItem it = items[i];
+        seg_dispatch<DT>(it.bucket, it.variant, in, out, off, list, it.first, it.count, lds_k, lds_p);
+    }
+}
+
+// Baseline: one lane group per segment, every segment in the tiling of the
+// maximum bucket (the reference's SortPartitions pattern).
+template <int DT, int B>
+__global__ void __launch_bounds__(kSegBlock)
+k_seg_plain(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* out, const int64_t* off,
+            int64_t segments) {
+    using E = Elem<DT>;
+    constexpr int LE = seg_loge<DT>(B, 0);
+    using C = TileCfg<E, B, LE>;
+    __shared__ typename E::Key lds_k[C::LDS_KEYS];
+    __shared__ int32_t lds_p[C::LDS_PAYLOAD];
+    const int tid = int(threadIdx.x);
+    const int group = tid >> C::LOGT;
+    const int t = tid & (C::T - 1);
+    const int64_t s = int64_t(blockIdx.x) * C::APB + group;
+    const bool active = s < segments;
+    const int64_t b0 = active ? off[s] : 0;
+    const int len = active ? int(off[s + 1] - b0) : 0;
+    tile_sort<E, B, LE>(in + b0, out + b0, len, t, lds_k + (C::NTRANS ? group * C::REGION : 0),
+                        lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
+}
+
+__global__ void k_seg_reset(int* counter);
+
+// Host side (segmented_<dtype>.hip).
+struct SegKernels {
+    hipError_t (*persistent)(const void* in, void* out, const int64_t* off, const int* list, const SegItem* items,
+                             int nitems, int* counter, int grid, hipStream_t s) = nullptr;
+    const void* persistent_fn = nullptr;
+    hipError_t (*plain)(const void* in, void* out, const int64_t* off, int64_t segments, int bucket,
+                        hipStream_t s) = nullptr;
+    int max_logw = 0;
+    int (*segments_per_item)(int bucket, int variant) = nullptr;
+    bool (*variant_ok)(int bucket, int variant) = nullptr;
+};
+
+template <int DT>
+hipError_t seg_launch_persistent(const void* in, void* out, const int64_t* off, const int* list, const SegItem* items,
+                                 int nitems, int* counter, int grid, hipStream_t s) {
+    using S = typename Elem<DT>::Storage;
+    hipLaunchKernelGGL(k_seg_persistent<DT>, dim3(unsigned(grid)), dim3(kSegBlock), 0, s,
+                       static_cast<const S*>(in), static_cast<S*>(out), off, list, items, nitems, counter);
+    return hipGetLastError();
+}
+
+template <int DT>
+hipError_t seg_launch_plain(const void* in, void* out, const int64_t* off, int64_t segments, int bucket,
+                            hipStream_t s) {
+    using S = typename Elem<DT>::Storage;
+    hipError_t err = hipErrorInvalidValue;
+    hfor<0, seg_max_logw<DT>() + 1>([&](auto b_) {
+        constexpr int B = decltype(b_)::value;
+        if (B == bucket) {
+            using C = TileCfg<Elem<DT>, B, seg_loge<DT>(B, 0)>;
+            const int64_t blocks = (segments + C::APB - 1) / C::APB;
+            if (blocks > 0)
+                hipLaunchKernelGGL((k_seg_plain<DT, B>), dim3(unsigned(blocks)), dim3(kSegBlock), 0, s,
+                                   static_cast<const S*>(in), static_cast<S*>(out), off, segments);
+            err = hipGetLastError();
+        }
+    });
+    return err;
+}
+
+template <int DT> int seg_spi(int b, int v) { return seg_segments_per_item<DT>(b, v); }
+template <int DT> bool seg_ok(int b, int v) { return seg_variant_ok<DT>(b, v); }
+
+template <int DT>
+SegKernels seg_kernels() {
+    SegKernels k;
+    k.persistent = &seg_launch_persistent<DT>;
+    k.persistent_fn = reinterpret_cast<const void*>(&k_seg_persistent<DT>);
+    k.plain = &seg_launch_plain<DT>;
+    k.max_logw = seg_max_logw<DT>();
+    k.segments_per_item = &seg_spi<DT>;
+    k.variant_ok = &seg_ok<DT>;
+    return k;
+}
+
+SegKernels seg_kernels_for(int dtype);
+
+}  // namespace cs

@@ -1,0 +1,382 @@
+// Ragged-batch plans: bucketing, work items, the hipGraph of one sort, and
+// the per-bucket tuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/cusort_hip.h"
+#include "gen.hpp"
+#include "segmented.hpp"
+
+namespace cs {
+
+__global__ void k_seg_reset(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+SegKernels seg_kernels_i32();
+SegKernels seg_kernels_u32();
+SegKernels seg_kernels_f32();
+SegKernels seg_kernels_u64();
+SegKernels seg_kernels_i64();
+SegKernels seg_kernels_rec32();
+SegKernels seg_kernels_rec64();
+
+SegKernels seg_kernels_for(int dtype) {
+    switch (dtype) {
+        case CS_I32: return seg_kernels_i32();
+        case CS_U32: return seg_kernels_u32();
+        case CS_F32: return seg_kernels_f32();
+        case CS_U64: return seg_kernels_u64();
+        case CS_I64: return seg_kernels_i64();
+        case CS_REC_I32_I32: return seg_kernels_rec32();
+        case CS_REC_I64_I32: return seg_kernels_rec64();
+        default: return SegKernels{};
+    }
+}
+
+namespace {
+
+constexpr int kMaxBuckets = 16;
+
+// Tuned variant per (dtype, bucket); -1 = not tuned (use variant 0).
+std::mutex g_tune_mu;
+int g_variant[8][kMaxBuckets];
+bool g_variant_init = false;
+
+int tuned_variant(int dtype, int b) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_variant_init) {
+        for (auto& row : g_variant)
+            for (int& v : row) v = -1;
+        g_variant_init = true;
+    }
+    return g_variant[dtype][b];
+}
+
+void set_tuned_variant(int dtype, int b, int v) {
+    tuned_variant(dtype, b);   // initialise
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_variant[dtype][b] = v;
+}
+
+int bucket_of(int64_t len) {
+    int b = 0;
+    while ((int64_t(1) << b) < len) ++b;
+    return b;
+}
+
+}  // namespace
+}  // namespace cs
+
+using namespace cs;
+
+struct cs_segplan {
+    int dtype = 0;
+    int64_t segments = 0;
+    int64_t total = 0;
+    int64_t max_len = 0;
+    int nitems = 0;
+    int grid = 0;
+    int variant[kMaxBuckets] = {};
+    int64_t bucket_segments[kMaxBuckets] = {};
+    SegKernels k;
+    int* d_list = nullptr;
+    SegItem* d_items = nullptr;
+    int* d_counter = nullptr;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t reset_node = nullptr;
+    hipGraphNode_t sort_node = nullptr;
+    // arguments baked into the graph's sort node
+    const void* g_in = nullptr;
+    void* g_out = nullptr;
+    const int64_t* g_off = nullptr;
+    void* args[7] = {};
+    void* reset_args[1] = {};
+    const void* a_in = nullptr;
+    void* a_out = nullptr;
+    const int64_t* a_off = nullptr;
+    const int* a_list = nullptr;
+    const SegItem* a_items = nullptr;
+    int a_nitems = 0;
+    int* a_counter = nullptr;
+};
+
+namespace {
+
+void free_plan(cs_segplan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_list) (void)hipFree(p->d_list);
+    if (p->d_items) (void)hipFree(p->d_items);
+    if (p->d_counter) (void)hipFree(p->d_counter);
+    delete p;
+}
+
+hipError_t build_graph(cs_segplan* p, const void* in, void* out, const int64_t* off) {
+    if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+    if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+    hipError_t e = hipGraphCreate(&p->graph, 0);
+    if (e != hipSuccess) return e;
+    // Counter reset is a kernel node, not a memset node.
+    p->a_counter = p->d_counter;
+    p->reset_args[0] = &p->a_counter;
+    hipKernelNodeParams rp{};
+    rp.func = reinterpret_cast<void*>(&k_seg_reset);
+    rp.gridDim = dim3(1);
+    rp.blockDim = dim3(64);
+    rp.sharedMemBytes = 0;
+    rp.kernelParams = p->reset_args;
+    rp.extra = nullptr;
+    e = hipGraphAddKernelNode(&p->reset_node, p->graph, nullptr, 0, &rp);
+    if (e != hipSuccess) return e;
+    p->a_in = in; p->a_out = out; p->a_off = off;
+    p->a_list = p->d_list; p->a_items = p->d_items; p->a_nitems = p->nitems;
+    p->args[0] = &p->a_in; p->args[1] = &p->a_out; p->args[2] = &p->a_off; p->args[3] = &p->a_list;
+    p->args[4] = &p->a_items; p->args[5] = &p->a_nitems; p->args[6] = &p->a_counter;
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void*>(p->k.persistent_fn);
+    kp.gridDim = dim3(unsigned(p->grid));
+    kp.blockDim = dim3(kSegBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = p->args;
+    kp.extra = nullptr;
+    e = hipGraphAddKernelNode(&p->sort_node, p->graph, &p->reset_node, 1, &kp);
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return e;
+    p->g_in = in; p->g_out = out; p->g_off = off;
+    return hipSuccess;
+}
+
+hipError_t update_graph(cs_segplan* p, const void* in, void* out, const int64_t* off) {
+    p->a_in = in; p->a_out = out; p->a_off = off;
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void*>(p->k.persistent_fn);
+    kp.gridDim = dim3(unsigned(p->grid));
+    kp.blockDim = dim3(kSegBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = p->args;
+    kp.extra = nullptr;
+    hipError_t e = hipGraphExecKernelNodeSetParams(p->exec, p->sort_node, &kp);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return build_graph(p, in, out, off);   // fall back to a fresh instantiation
+    }
+    p->g_in = in; p->g_out = out; p->g_off = off;
+    return hipSuccess;
+}
+
+int make_plan_impl(cs_segplan** out, int dtype, const int64_t* h_offsets, int64_t segments, uint32_t flags,
+                   const int* force_variant) {
+    if (!out || segments < 0 || (segments > 0 && !h_offsets)) return CS_E_INVALID_VALUE;
+    if (dtype < CS_I32 || dtype > CS_I64) return CS_E_INVALID_VALUE;
+    if (flags & ~(CS_SEG_NO_GRAPH)) return CS_E_INVALID_VALUE;
+    if (segments > (int64_t(1) << 31) - 1) return CS_E_INVALID_VALUE;
+    cs_segplan* p = new cs_segplan();
+    p->dtype = dtype;
+    p->segments = segments;
+    p->k = seg_kernels_for(dtype);
+    p->use_graph = (flags & CS_SEG_NO_GRAPH) == 0;
+    const int maxb = p->k.max_logw;
+    // bucket the segments
+    std::vector<std::vector<int>> per(kMaxBuckets);
+    for (int64_t s = 0; s < segments; ++s) {
+        const int64_t len = h_offsets[s + 1] - h_offsets[s];
+        if (len < 0) { delete p; return CS_E_INVALID_VALUE; }
+        if (len > (int64_t(1) << maxb)) { delete p; return CS_E_CAPACITY; }
+        p->max_len = std::max(p->max_len, len);
+        const int b = bucket_of(len);
+        per[size_t(b)].push_back(int(s));
+    }
+    p->total = segments ? h_offsets[segments] - h_offsets[0] : 0;
+    std::vector<int> list;
+    std::vector<SegItem> items;
+    list.reserve(size_t(segments));
+    for (int b = maxb; b >= 0; --b) {
+        const auto& seg = per[size_t(b)];
+        p->bucket_segments[b] = int64_t(seg.size());
+        int v = force_variant ? force_variant[b] : tuned_variant(dtype, b);
+        if (v < 0 || !p->k.variant_ok(b, v)) v = 0;
+        if (!p->k.variant_ok(b, v)) v = 2;
+        p->variant[b] = v;
+        if (seg.empty()) continue;
+        const int spi = p->k.segments_per_item(b, v);
+        for (size_t i = 0; i < seg.size(); i += size_t(spi)) {
+            SegItem it;
+            it.bucket = b;
+            it.variant = v;
+            it.first = int(list.size() + i);
+            it.count = int(std::min(seg.size() - i, size_t(spi)));
+            items.push_back(it);
+        }
+        list.insert(list.end(), seg.begin(), seg.end());
+    }
+    p->nitems = int(items.size());
+    int dev = 0, cus = 1, per_cu = 1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->k.persistent_fn, kSegBlock, 0);
+    if (e != hipSuccess) { free_plan(p); return int(e); }
+    p->grid = std::max(1, std::min(cus * std::max(per_cu, 1), std::max(p->nitems, 1)));
+    e = hipMalloc(&p->d_list, std::max<size_t>(list.size(), 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_items, std::max<size_t>(items.size(), 1) * sizeof(SegItem));
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e == hipSuccess && !list.empty())
+        e = hipMemcpy(p->d_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpy(p->d_items, items.data(), items.size() * sizeof(SegItem), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { free_plan(p); return int(e); }
+    *out = p;
+    return CS_OK;
+}
+
+int run_plan(cs_segplan* p, const void* in, void* out, const int64_t* d_offsets, hipStream_t stream) {
+    if (!p) return CS_E_INVALID_VALUE;
+    if (p->nitems == 0) return CS_OK;
+    if (!in || !out || !d_offsets) return CS_E_INVALID_VALUE;
+    hipError_t e;
+    if (p->use_graph) {
+        if (!p->exec) e = build_graph(p, in, out, d_offsets);
+        else if (p->g_in != in || p->g_out != out || p->g_off != d_offsets) e = update_graph(p, in, out, d_offsets);
+        else e = hipSuccess;
+        if (e == hipSuccess) e = hipGraphLaunch(p->exec, stream);
+    } else {
+        hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, stream, p->d_counter);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = p->k.persistent(in, out, d_offsets, p->d_list, p->d_items, p->nitems, p->d_counter, p->grid, stream);
+    }
+    return e == hipSuccess ? CS_OK : int(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cs_segmented_plan_create(cs_segplan** plan, int dtype, const int64_t* h_offsets, int64_t segments,
+                             uint32_t flags) {
+    return make_plan_impl(plan, dtype, h_offsets, segments, flags, nullptr);
+}
+
+int cs_segmented_plan_destroy(cs_segplan* plan) {
+    free_plan(plan);
+    return CS_OK;
+}
+
+int cs_segmented_sort(cs_segplan* plan, const void* in, void* out, const int64_t* d_offsets, hipStream_t stream) {
+    return run_plan(plan, in, out, d_offsets, stream);
+}
+
+int cs_segmented_plan_info(const cs_segplan* plan, int32_t* work_items, int32_t* grid, int64_t* max_len,
+                           int32_t* variants, int64_t* bucket_segments, int32_t buckets) {
+    if (!plan) return CS_E_INVALID_VALUE;
+    if (work_items) *work_items = plan->nitems;
+    if (grid) *grid = plan->grid;
+    if (max_len) *max_len = plan->max_len;
+    for (int b = 0; b < buckets && b < kMaxBuckets; ++b) {
+        if (variants) variants[b] = plan->variant[b];
+        if (bucket_segments) bucket_segments[b] = plan->bucket_segments[b];
+    }
+    return CS_OK;
+}
+
+int32_t cs_segmented_max_len(int dtype) {
+    if (dtype < CS_I32 || dtype > CS_I64) return 0;
+    return int32_t(1) << seg_kernels_for(dtype).max_logw;
+}
+
+int cs_segmented_sort_plain(int dtype, const void* in, void* out, const int64_t* d_offsets, int64_t segments,
+                            int32_t max_len, hipStream_t stream) {
+    if (dtype < CS_I32 || dtype > CS_I64 || segments < 0 || max_len < 0) return CS_E_INVALID_VALUE;
+    if (segments == 0) return CS_OK;
+    if (!in || !out || !d_offsets) return CS_E_INVALID_VALUE;
+    const SegKernels k = seg_kernels_for(dtype);
+    if (max_len > (int32_t(1) << k.max_logw)) return CS_E_CAPACITY;
+    const hipError_t e = k.plain(in, out, d_offsets, segments, bucket_of(max_len), stream);
+    return e == hipSuccess ? CS_OK : int(e);
+}
+
+int cs_segmented_get_variant(int dtype, int32_t bucket, int32_t* variant) {
+    if (dtype < CS_I32 || dtype > CS_I64 || bucket < 0 || bucket >= kMaxBuckets || !variant) return CS_E_INVALID_VALUE;
+    *variant = tuned_variant(dtype, bucket);
+    return CS_OK;
+}
+
+int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant) {
+    if (dtype < CS_I32 || dtype > CS_I64 || bucket < 0 || bucket >= kMaxBuckets) return CS_E_INVALID_VALUE;
+    if (variant != -1 && !seg_kernels_for(dtype).variant_ok(bucket, variant)) return CS_E_UNSUPPORTED;
+    set_tuned_variant(dtype, bucket, variant);
+    return CS_OK;
+}
+
+// Tuner: for every bucket, time each variant on a batch of full-length
+// segments (2^b elements each, ~2^22 elements in total, uniform keys from
+// the library's generator) through the persistent kernel, keep the fastest
+// (median of `trials` timed runs).  Allocates scratch; not for timed paths.
+int cs_segmented_tune(int dtype, int32_t trials, hipStream_t stream) {
+    if (dtype < CS_I32 || dtype > CS_I64 || trials < 1) return CS_E_INVALID_VALUE;
+    const SegKernels k = seg_kernels_for(dtype);
+    const size_t esz = cs_dtype_size(dtype);
+    const int64_t budget = int64_t(1) << 22;
+    void *in = nullptr, *out = nullptr;
+    int64_t* d_off = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipError_t e = hipMalloc(&in, size_t(budget) * esz);
+    if (e == hipSuccess) e = hipMalloc(&out, size_t(budget) * esz);
+    if (e == hipSuccess) e = hipMalloc(&d_off, size_t(budget + 1) * sizeof(int64_t));
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) e = generate(dtype, CS_DIST_UNIFORM, budget, budget, 12345, in, stream);
+    int rc = e == hipSuccess ? CS_OK : int(e);
+    for (int b = 1; b <= k.max_logw && rc == CS_OK; ++b) {
+        const int64_t len = int64_t(1) << b;
+        const int64_t segs = budget / len;
+        std::vector<int64_t> off(size_t(segs) + 1);
+        for (int64_t s = 0; s <= segs; ++s) off[size_t(s)] = s * len;
+        e = hipMemcpy(d_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { rc = int(e); break; }
+        float best = 0;
+        int best_v = -1;
+        for (int v = 0; v < kSegVariants; ++v) {
+            if (!k.variant_ok(b, v)) continue;
+            int force[kMaxBuckets];
+            for (int& f : force) f = v;
+            cs_segplan* p = nullptr;
+            rc = make_plan_impl(&p, dtype, off.data(), segs, CS_SEG_NO_GRAPH, force);
+            if (rc != CS_OK) break;
+            std::vector<float> ms;
+            for (int r = 0; r <= trials && rc == CS_OK; ++r) {
+                (void)hipEventRecord(t0, stream);
+                rc = run_plan(p, in, out, d_off, stream);
+                (void)hipEventRecord(t1, stream);
+                (void)hipEventSynchronize(t1);
+                float m = 0;
+                (void)hipEventElapsedTime(&m, t0, t1);
+                if (r > 0) ms.push_back(m);   // first run warms up
+            }
+            free_plan(p);
+            if (rc != CS_OK) break;
+            std::sort(ms.begin(), ms.end());
+            const float med = ms[ms.size() / 2];
+            if (best_v < 0 || med < best) { best = med; best_v = v; }
+        }
+        if (rc == CS_OK && best_v >= 0) set_tuned_variant(dtype, b, best_v);
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
+    if (d_off) (void)hipFree(d_off);
+    return rc;
+}
+
+}  // extern "C"

@@ -73,6 +73,25 @@ def lib():
         L.cs_batched_max_n.argtypes = [ctypes.c_int]
         L.cs_batched_describe.restype = ctypes.c_int
         L.cs_batched_describe.argtypes = [ctypes.c_int, i32, ctypes.c_int, i32, u32, P32, P32, P32, P32, P32]
+        P64 = ctypes.POINTER(ctypes.c_int64)
+        L.cs_segmented_plan_create.restype = ctypes.c_int
+        L.cs_segmented_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, vp, i64, u32]
+        L.cs_segmented_sort.restype = ctypes.c_int
+        L.cs_segmented_sort.argtypes = [vp, vp, vp, vp, vp]
+        L.cs_segmented_plan_destroy.restype = ctypes.c_int
+        L.cs_segmented_plan_destroy.argtypes = [vp]
+        L.cs_segmented_plan_info.restype = ctypes.c_int
+        L.cs_segmented_plan_info.argtypes = [vp, P32, P32, P64, P32, P64, i32]
+        L.cs_segmented_max_len.restype = i32
+        L.cs_segmented_max_len.argtypes = [ctypes.c_int]
+        L.cs_segmented_sort_plain.restype = ctypes.c_int
+        L.cs_segmented_sort_plain.argtypes = [ctypes.c_int, vp, vp, vp, i64, i32, vp]
+        L.cs_segmented_tune.restype = ctypes.c_int
+        L.cs_segmented_tune.argtypes = [ctypes.c_int, i32, vp]
+        L.cs_segmented_get_variant.restype = ctypes.c_int
+        L.cs_segmented_get_variant.argtypes = [ctypes.c_int, i32, P32]
+        L.cs_segmented_set_variant.restype = ctypes.c_int
+        L.cs_segmented_set_variant.argtypes = [ctypes.c_int, i32, i32]
         L.cs_device_copy.restype = ctypes.c_int
         L.cs_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.cs_generate.restype = ctypes.c_int
@@ -128,6 +147,78 @@ def generate(dtype: int, dist: int, n: int, count: int, seed: int, dst: int, str
 
 def device_copy(dst: int, src: int, nbytes: int, stream: int = 0) -> None:
     check(lib().cs_device_copy(dst, src, nbytes, stream), "cs_device_copy")
+
+
+SEG_NO_GRAPH = 1
+
+
+class SegPlan:
+    """A ragged-batch plan (cs_segplan) built from host CSR offsets."""
+
+    def __init__(self, dtype: int, offsets, flags: int = 0):
+        import numpy as np
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.dtype = dtype
+        self.segments = off.size - 1
+        h = ctypes.c_void_p()
+        check(lib().cs_segmented_plan_create(ctypes.byref(h), dtype, off.ctypes.data, self.segments, flags),
+              "cs_segmented_plan_create")
+        self.handle = h.value
+
+    def sort(self, src: int, dst: int, d_offsets: int, stream: int = 0) -> None:
+        check(lib().cs_segmented_sort(self.handle, src, dst, d_offsets, stream), "cs_segmented_sort")
+
+    def info(self) -> dict:
+        items, grid = ctypes.c_int32(0), ctypes.c_int32(0)
+        mx = ctypes.c_int64(0)
+        var = (ctypes.c_int32 * 16)()
+        cnt = (ctypes.c_int64 * 16)()
+        check(lib().cs_segmented_plan_info(self.handle, ctypes.byref(items), ctypes.byref(grid), ctypes.byref(mx),
+                                           var, cnt, 16), "cs_segmented_plan_info")
+        return {"work_items": items.value, "grid": grid.value, "max_len": mx.value,
+                "variants": list(var), "bucket_segments": list(cnt)}
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.cs_segmented_plan_destroy(self.handle)
+            self.handle = None
+
+
+def segmented_sort_plain(dtype: int, src: int, dst: int, d_offsets: int, segments: int, max_len: int,
+                         stream: int = 0) -> None:
+    check(lib().cs_segmented_sort_plain(dtype, src, dst, d_offsets, segments, max_len, stream),
+          "cs_segmented_sort_plain")
+
+
+def segmented_max_len(dtype: int) -> int:
+    return int(lib().cs_segmented_max_len(dtype))
+
+
+def segmented_tune(dtype: int, trials: int = 3, stream: int = 0) -> None:
+    check(lib().cs_segmented_tune(dtype, trials, stream), "cs_segmented_tune")
+
+
+def segmented_variant(dtype: int, bucket: int) -> int:
+    v = ctypes.c_int32(0)
+    check(lib().cs_segmented_get_variant(dtype, bucket, ctypes.byref(v)), "cs_segmented_get_variant")
+    return int(v.value)
+
+
+def segmented_set_variant(dtype: int, bucket: int, variant: int) -> int:
+    return int(lib().cs_segmented_set_variant(dtype, bucket, variant))
+
+
+def ragged_lengths(segments: int = 100000, max_len: int = 8192, seed: int = 42):
+    """Lengths floor(exp(U[0, ln(max_len+1)))) clipped to [1, max_len] (log-uniform)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lens = np.floor(np.exp(rng.uniform(0.0, np.log(max_len + 1.0), segments))).astype(np.int64)
+    return np.clip(lens, 1, max_len)
+
+
+def offsets_from_lengths(lens):
+    import numpy as np
+    return np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
 
 
 # ------------------------------------------------------------ buffers

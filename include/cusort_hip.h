@@ -129,6 +129,43 @@ int cs_batched_describe(int dtype, int32_t n, int method, int32_t threads, uint3
                         int32_t* block_threads, int32_t* items_per_thread, int32_t* lds_bytes,
                         int32_t* vgprs, int32_t* arrays_per_block);
 
+/* ------------------------------------------------------------ ragged batch
+ * Sort every segment s = [offsets[s], offsets[s+1]) of a ragged batch given
+ * as CSR offsets (int64, non-decreasing), each segment independently.
+ * No reference counterpart as an API: the reference sorts ragged buckets
+ * only inside its partition sort (SortPartitions, cusort.hpp:1802-1846).
+ *
+ * A plan is built once on the host offsets: segments are bucketed by
+ * ceil(log2(length)), each bucket gets the tiling its tuned variant names,
+ * work items are ordered largest bucket first, and one sort is one graph
+ * (counter-reset kernel + one persistent kernel) launched on `stream`.
+ * The plan keeps its graph; calls with other pointers update the graph's
+ * kernel arguments.  One plan per host thread (plans are not thread-safe).
+ * Longest segment: cs_segmented_max_len(dtype) (8192 for 4-byte keys,
+ * 4096 otherwise). */
+typedef struct cs_segplan cs_segplan;
+#define CS_SEG_NO_GRAPH 1u         /* launch the kernels directly, no hipGraph */
+int cs_segmented_plan_create(cs_segplan** plan, int dtype, const int64_t* h_offsets, int64_t segments,
+                             uint32_t flags);
+int cs_segmented_sort(cs_segplan* plan, const void* in, void* out, const int64_t* d_offsets, hipStream_t stream);
+int cs_segmented_plan_destroy(cs_segplan* plan);
+/* Work items, persistent grid size, longest segment, and per bucket b < buckets
+ * the chosen variant (0: 32-key register tile, 1: 16-key tile, 2: LDS network)
+ * and its segment count.  Any pointer may be NULL. */
+int cs_segmented_plan_info(const cs_segplan* plan, int32_t* work_items, int32_t* grid, int64_t* max_len,
+                           int32_t* variants, int64_t* bucket_segments, int32_t buckets);
+int32_t cs_segmented_max_len(int dtype);
+/* Baseline: one lane group per segment, every segment in the tiling of the
+ * longest one (the reference's SortPartitions pattern), one plain launch. */
+int cs_segmented_sort_plain(int dtype, const void* in, void* out, const int64_t* d_offsets, int64_t segments,
+                            int32_t max_len, hipStream_t stream);
+/* Autotuner: time every variant of every bucket on full-length segments and
+ * keep the fastest for plans created afterwards (per process, per dtype).
+ * get/set read or override the table (-1 = untuned). */
+int cs_segmented_tune(int dtype, int32_t trials, hipStream_t stream);
+int cs_segmented_get_variant(int dtype, int32_t bucket, int32_t* variant);
+int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant);
+
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel (16-byte
  * accesses when both pointers are 16-byte aligned).  The reference harness
  * restores every input with a device copy before each trial
