@@ -37,7 +37,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_i32",
-                    choices=["batched_i32", "batched_rec32", "copy", "ragged_i32", "ragged_f32"])
+                    choices=["batched_i32", "batched_rec32", "copy", "ragged_i32", "ragged_f32",
+                             "large_i32", "large_u64"])
+    ap.add_argument("--large-method", default="hybrid", choices=["hybrid", "merge", "partition"])
+    ap.add_argument("--large-log2", type=int, default=0, help="log2 of the array length (0: 28 for int32, 27 for uint64)")
+    ap.add_argument("--large-dist", default="uniform", choices=["uniform", "presorted"])
     ap.add_argument("--segments", type=int, default=100000)
     ap.add_argument("--max-len", type=int, default=8192)
     ap.add_argument("--no-tune", action="store_true")
@@ -233,6 +237,68 @@ def bench_ragged(args, dtype_name: str):
             "cpu_baseline": cpu, "validated": ok}
 
 
+def bench_large(args, dtype_name: str):
+    """One array of 2^28 int32 or 2^27 uint64 keys (uniform, or 10% presorted
+    4096-runs), sorted in place by the hybrid / merge / partition path.  Each
+    step restores the pristine input (untimed, as bench.cu:335) and times the
+    sort alone with events on the sort's stream."""
+    import cusort_hip as cs
+    import hiprt
+
+    dt = cs.DTYPE_NAMES[dtype_name]
+    log2n = args.large_log2 or (28 if dt in (cs.I32, cs.U32, cs.F32) else 27)
+    n = 1 << log2n
+    dist = cs.D_UNIFORM if args.large_dist == "uniform" else cs.D_PRESORTED_RUNS
+    hiprt.set_device(0)
+    stream = hiprt.Stream()
+    pristine, work = cs.device_buffer(dt, n), cs.device_buffer(dt, n)
+    cs.generate(dt, dist, n, n, 42, pristine.ptr, stream.handle)
+    ws = cs.Workspace(dt, n)
+    nbytes = n * cs.DTYPE_SIZE[dt]
+
+    def sort():
+        if args.large_method == "hybrid":
+            cs.sort_hybrid(dt, work.ptr, n, ws, stream.handle)
+        elif args.large_method == "merge":
+            cs.sort_merge_ws(dt, work.ptr, n, ws, stream.handle)
+        else:
+            cs.sort_partition(dt, work.ptr, n, stream=stream.handle)
+
+    for _ in range(args.warmup):
+        cs.device_copy(work.ptr, pristine.ptr, nbytes, stream.handle)
+        sort()
+    hiprt.synchronize()
+    total = 0.0
+    for _ in range(args.steps):
+        cs.device_copy(work.ptr, pristine.ptr, nbytes, stream.handle)
+        start, end = hiprt.Event(), hiprt.Event()
+        start.record(stream)
+        sort()
+        end.record(stream)
+        end.synchronize()
+        total += start.elapsed_ms(end)
+    ms = total / args.steps
+    ok, cpu = None, None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        host_in = oracle.generate(dt, dist, n, n, 42)
+        got = cs.to_numpy(work, dt, n)
+        ok = got.tobytes() == oracle.sort_large(dt, host_in).tobytes()
+        secs, threads = oracle.time_large(dt, host_in, threads=0, trials=1)
+        cpu = {"value": n / secs / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
+               "sample": f"the full array (2^{log2n} keys), __gnu_parallel::sort over {threads} host threads, 1 trial"}
+    alg = 2 * nbytes
+    return {"metric": METRIC, "value": n / (ms * 1e-3) / 1e9, "unit": "Gkeys/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32" if dt == cs.I32 else "uint64", "data": "synthetic",
+            "config": {"workload": f"large-array sort, 2^{log2n} {dtype_name} keys, {args.large_dist}, seed 42, "
+                                   f"{args.large_method} path", "n": n, "method": args.large_method},
+            "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+            "cpu_baseline": cpu, "validated": ok}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -244,6 +310,8 @@ def main():
         line = bench_batched(args, "i32")
     elif args.workload == "copy":
         line = bench_copy(args)
+    elif args.workload.startswith("large"):
+        line = bench_large(args, args.workload.split("_")[1])
     elif args.workload.startswith("ragged"):
         line = bench_ragged(args, args.workload.split("_")[1])
     else:

@@ -1,0 +1,204 @@
+// Large-array sorts: SortMerge / HybridSort / Sort entry points and the
+// reusable workspace (the reference's CuSortWorkspace, cusort.hpp:67-219).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/cusort_hip.h"
+#include "gen.hpp"
+#include "large.hpp"
+
+namespace cs {
+
+LargeKernels large_kernels_i32();
+LargeKernels large_kernels_u32();
+LargeKernels large_kernels_f32();
+LargeKernels large_kernels_u64();
+LargeKernels large_kernels_i64();
+LargeKernels large_kernels_rec32();
+LargeKernels large_kernels_rec64();
+
+LargeKernels large_kernels_for(int dtype) {
+    switch (dtype) {
+        case CS_I32: return large_kernels_i32();
+        case CS_U32: return large_kernels_u32();
+        case CS_F32: return large_kernels_f32();
+        case CS_U64: return large_kernels_u64();
+        case CS_I64: return large_kernels_i64();
+        case CS_REC_I32_I32: return large_kernels_rec32();
+        case CS_REC_I64_I32: return large_kernels_rec64();
+        default: return LargeKernels{};
+    }
+}
+
+int partition_sort(int dtype, void* data, int64_t n, void* out, void* scratch, size_t scratch_elems, int max_parts,
+                   hipStream_t stream);
+
+}  // namespace cs
+
+using namespace cs;
+
+struct cs_workspace {
+    int dtype = 0;
+    void* values = nullptr;
+    size_t capacity = 0;
+    int blocks_per_cu = 4;
+    int streams = 8;
+};
+
+namespace {
+
+bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_I64; }
+
+// Chunk sort + merge passes.  Result in `final` (== data for in place).
+// `scratch` holds at least n elements and may be `data` itself for an
+// out-of-place call (the input is then consumed, as in the reference).
+int merge_sort(int dtype, void* data, int64_t n, void* final, void* scratch, hipStream_t stream) {
+    const LargeKernels k = large_kernels_for(dtype);
+    if (n <= k.chunk) {
+        const int rc = cs_batched_sort(dtype, data, final, 1, int32_t(n), CS_METHOD_AUTO, 0, 0, stream);
+        return rc;
+    }
+    int passes = 0;
+    for (int64_t run = k.chunk; run < n; run *= 2) ++passes;
+    // The chunk sort lands where an even number of ping-pong passes ends in `final`.
+    void* a = (passes % 2 == 0) ? final : scratch;
+    void* b = (a == final) ? scratch : final;
+    hipError_t e = k.chunk_sort(data, a, n, stream);
+    for (int64_t run = k.chunk; e == hipSuccess && run < n; run *= 2) {
+        e = k.merge_pass(a, b, n, run, stream);
+        std::swap(a, b);
+    }
+    return e == hipSuccess ? CS_OK : int(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cs_workspace_create(cs_workspace** ws, int dtype, size_t capacity, int32_t blocks_per_cu, int32_t n_streams) {
+    if (!ws || !valid_dtype(dtype) || blocks_per_cu < 1 || n_streams < 1) return CS_E_INVALID_VALUE;
+    cs_workspace* w = new cs_workspace();
+    w->dtype = dtype;
+    w->blocks_per_cu = blocks_per_cu;
+    w->streams = n_streams;
+    *ws = w;
+    if (capacity) {
+        const int rc = cs_workspace_reserve(w, capacity);
+        if (rc != CS_OK) { delete w; *ws = nullptr; return rc; }
+    }
+    return CS_OK;
+}
+
+int cs_workspace_reserve(cs_workspace* ws, size_t capacity) {
+    if (!ws) return CS_E_INVALID_VALUE;
+    if (capacity <= ws->capacity) return CS_OK;
+    if (ws->values) {
+        hipError_t e = hipDeviceSynchronize();   // the old buffer may still be in use by queued work
+        if (e == hipSuccess) e = hipFree(ws->values);
+        ws->values = nullptr;
+        ws->capacity = 0;
+        if (e != hipSuccess) return int(e);
+    }
+    const hipError_t e = hipMalloc(&ws->values, capacity * cs_dtype_size(ws->dtype));
+    if (e != hipSuccess) return int(e);
+    ws->capacity = capacity;
+    return CS_OK;
+}
+
+int cs_workspace_capacity(const cs_workspace* ws, size_t* capacity) {
+    if (!ws || !capacity) return CS_E_INVALID_VALUE;
+    *capacity = ws->capacity;
+    return CS_OK;
+}
+
+int cs_workspace_destroy(cs_workspace* ws) {
+    if (!ws) return CS_OK;
+    if (ws->values) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ws->values);
+    }
+    delete ws;
+    return CS_OK;
+}
+
+int cs_sort_merge(int dtype, void* data, int64_t n, int32_t tile, int32_t merge_threads, void* ext_buf,
+                  size_t ext_elems, int stored_in_right_place, hipStream_t stream) {
+    if (!valid_dtype(dtype) || n < 0) return CS_E_INVALID_VALUE;
+    if (tile != 0 && (tile < 32 || (tile & (tile - 1)) != 0)) return CS_E_INVALID_VALUE;
+    if (merge_threads < 0 || merge_threads > 1024) return CS_E_INVALID_VALUE;
+    if (!stored_in_right_place && (!ext_buf || ext_elems < size_t(n))) return CS_E_INVALID_VALUE;
+    if (n == 0) return CS_OK;
+    if (!data) return CS_E_INVALID_VALUE;
+    void* final = stored_in_right_place ? data : ext_buf;
+    if (n == 1) {
+        if (final == data) return CS_OK;
+        return cs_device_copy(final, data, cs_dtype_size(dtype), stream);
+    }
+    if (!stored_in_right_place) return merge_sort(dtype, data, n, final, data, stream);
+    if (ext_buf && ext_elems >= size_t(n)) return merge_sort(dtype, data, n, data, ext_buf, stream);
+    // No usable buffer: a stream-ordered temporary (cudaMallocAsync in the reference).
+    void* tmp = nullptr;
+    hipError_t e = hipMallocAsync(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
+    if (e != hipSuccess) return int(e);
+    const int rc = merge_sort(dtype, data, n, data, tmp, stream);
+    e = hipFreeAsync(tmp, stream);
+    return rc != CS_OK ? rc : (e == hipSuccess ? CS_OK : int(e));
+}
+
+int cs_sort_merge_ws(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream) {
+    if (!ws || ws->dtype != dtype) return CS_E_WORKSPACE;
+    if (n < 0) return CS_E_INVALID_VALUE;
+    if (n > 1 && ws->capacity < size_t(n)) return CS_E_WORKSPACE;
+    return cs_sort_merge(dtype, data, n, 0, 0, ws->values, ws->capacity, 1, stream);
+}
+
+int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32_t max_parts, uint32_t flags,
+                      hipStream_t stream) {
+    if (!valid_dtype(dtype) || n < 0) return CS_E_INVALID_VALUE;
+    if (min_parts < 0 || max_parts < 0 || (max_parts && min_parts > max_parts)) return CS_E_INVALID_VALUE;
+    if (flags & ~(CS_FLAG_TESTSORTED | CS_FLAG_FULLSM | CS_FLAG_ASYNC)) return CS_E_INVALID_VALUE;
+    if (n <= 1) return CS_OK;
+    if (!data) return CS_E_INVALID_VALUE;
+    void* tmp = nullptr;
+    hipError_t e = hipMallocAsync(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
+    if (e != hipSuccess) return int(e);
+    const int rc = partition_sort(dtype, data, n, data, tmp, size_t(n), max_parts, stream);
+    e = hipFreeAsync(tmp, stream);
+    return rc != CS_OK ? rc : (e == hipSuccess ? CS_OK : int(e));
+}
+
+int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream) {
+    if (!valid_dtype(dtype) || n < 0) return CS_E_INVALID_VALUE;
+    if (ws && ws->dtype != dtype) return CS_E_WORKSPACE;
+    if (n <= 1) return CS_OK;
+    if (!data) return CS_E_INVALID_VALUE;
+    const int64_t threshold = cs_hybrid_threshold(dtype);
+    void* scratch = nullptr;
+    bool owned = false;
+    if (ws && ws->capacity >= size_t(n)) {
+        scratch = ws->values;
+    } else {
+        const hipError_t e = hipMallocAsync(&scratch, size_t(n) * cs_dtype_size(dtype), stream);
+        if (e != hipSuccess) return int(e);
+        owned = true;
+    }
+    int rc;
+    if (n < threshold) rc = merge_sort(dtype, data, n, data, scratch, stream);
+    else rc = partition_sort(dtype, data, n, data, scratch, size_t(n), 0, stream);
+    if (owned) {
+        const hipError_t e = hipFreeAsync(scratch, stream);
+        if (rc == CS_OK && e != hipSuccess) rc = int(e);
+    }
+    return rc;
+}
+
+int64_t cs_hybrid_threshold(int dtype) {
+    // Measured crossover on MI355X; the reference encodes its own as
+    // CuSortTuning (cusort.hpp:2476-2483: never for 4-byte keys, 2^27 for
+    // 8-byte structures).
+    if (!valid_dtype(dtype)) return 0;
+    return cs_dtype_size(dtype) <= 4 ? INT64_MAX : (int64_t(1) << 27);
+}
+
+}  // extern "C"

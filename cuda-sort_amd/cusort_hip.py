@@ -92,6 +92,25 @@ def lib():
         L.cs_segmented_get_variant.argtypes = [ctypes.c_int, i32, P32]
         L.cs_segmented_set_variant.restype = ctypes.c_int
         L.cs_segmented_set_variant.argtypes = [ctypes.c_int, i32, i32]
+        sz = ctypes.c_size_t
+        L.cs_workspace_create.restype = ctypes.c_int
+        L.cs_workspace_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, sz, i32, i32]
+        L.cs_workspace_reserve.restype = ctypes.c_int
+        L.cs_workspace_reserve.argtypes = [vp, sz]
+        L.cs_workspace_capacity.restype = ctypes.c_int
+        L.cs_workspace_capacity.argtypes = [vp, ctypes.POINTER(sz)]
+        L.cs_workspace_destroy.restype = ctypes.c_int
+        L.cs_workspace_destroy.argtypes = [vp]
+        L.cs_sort_merge.restype = ctypes.c_int
+        L.cs_sort_merge.argtypes = [ctypes.c_int, vp, i64, i32, i32, vp, sz, ctypes.c_int, vp]
+        L.cs_sort_merge_ws.restype = ctypes.c_int
+        L.cs_sort_merge_ws.argtypes = [ctypes.c_int, vp, i64, vp, vp]
+        L.cs_sort_partition.restype = ctypes.c_int
+        L.cs_sort_partition.argtypes = [ctypes.c_int, vp, i64, i32, i32, u32, vp]
+        L.cs_sort_hybrid.restype = ctypes.c_int
+        L.cs_sort_hybrid.argtypes = [ctypes.c_int, vp, i64, vp, vp]
+        L.cs_hybrid_threshold.restype = i64
+        L.cs_hybrid_threshold.argtypes = [ctypes.c_int]
         L.cs_device_copy.restype = ctypes.c_int
         L.cs_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.cs_generate.restype = ctypes.c_int
@@ -150,6 +169,53 @@ def device_copy(dst: int, src: int, nbytes: int, stream: int = 0) -> None:
 
 
 SEG_NO_GRAPH = 1
+
+
+class Workspace:
+    """cs_workspace: reusable scratch for the large-array sorts."""
+
+    def __init__(self, dtype: int, capacity: int = 0, blocks_per_cu: int = 4, n_streams: int = 8):
+        h = ctypes.c_void_p()
+        check(lib().cs_workspace_create(ctypes.byref(h), dtype, capacity, blocks_per_cu, n_streams),
+              "cs_workspace_create")
+        self.handle = h.value
+        self.dtype = dtype
+
+    def reserve(self, capacity: int) -> None:
+        check(lib().cs_workspace_reserve(self.handle, capacity), "cs_workspace_reserve")
+
+    def capacity(self) -> int:
+        c = ctypes.c_size_t(0)
+        check(lib().cs_workspace_capacity(self.handle, ctypes.byref(c)), "cs_workspace_capacity")
+        return int(c.value)
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.cs_workspace_destroy(self.handle)
+            self.handle = None
+
+
+def sort_merge(dtype: int, data: int, n: int, tile: int = 0, merge_threads: int = 0, ext: int = 0,
+               ext_elems: int = 0, in_place: bool = True, stream: int = 0) -> None:
+    check(lib().cs_sort_merge(dtype, data, n, tile, merge_threads, ext or None, ext_elems, 1 if in_place else 0,
+                              stream), "cs_sort_merge")
+
+
+def sort_merge_ws(dtype: int, data: int, n: int, ws: "Workspace", stream: int = 0) -> None:
+    check(lib().cs_sort_merge_ws(dtype, data, n, ws.handle, stream), "cs_sort_merge_ws")
+
+
+def sort_partition(dtype: int, data: int, n: int, min_parts: int = 0, max_parts: int = 0, flags: int = 0,
+                   stream: int = 0) -> None:
+    check(lib().cs_sort_partition(dtype, data, n, min_parts, max_parts, flags, stream), "cs_sort_partition")
+
+
+def sort_hybrid(dtype: int, data: int, n: int, ws: "Workspace | None" = None, stream: int = 0) -> None:
+    check(lib().cs_sort_hybrid(dtype, data, n, ws.handle if ws else None, stream), "cs_sort_hybrid")
+
+
+def hybrid_threshold(dtype: int) -> int:
+    return int(lib().cs_hybrid_threshold(dtype))
 
 
 class SegPlan:

@@ -166,6 +166,42 @@ int cs_segmented_tune(int dtype, int32_t trials, hipStream_t stream);
 int cs_segmented_get_variant(int dtype, int32_t bucket, int32_t* variant);
 int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant);
 
+/* ------------------------------------------------------------ large arrays
+ * One array of n elements, sorted in place unless stated.
+ *
+ * cs_sort_merge replaces SortMerge<T, MaxValuesToSort, TestIfSorted, FullSM>
+ * (cusort.hpp:2161-2300): chunks of 8192 (4-byte) / 4096 (8- and 16-byte)
+ * elements are sorted by the register-tile kernel, then merge-path passes
+ * double the sorted runs.  `tile` (MaxValuesToSort, power of two or 0) and
+ * `merge_threads` are accepted for call compatibility; the chunk and merge
+ * tile sizes are the library's.  ext_buf/ext_elems: optional scratch of at
+ * least n elements; stored_in_right_place = 0 writes the result to ext_buf
+ * (then required) and, as in the reference, uses the input as scratch.
+ * Without scratch a stream-ordered temporary is allocated (hipMallocAsync).
+ *
+ * cs_sort_partition replaces Sort<T, TestIfSorted, FullSM, ASyncCpyBito,
+ * MinNbParts, MaxNbParts> (cusort.hpp:2020-2023, 1869-2017): sample-sort
+ * partition into up to 2*max_parts-1 buckets (equality buckets for repeated
+ * splitters), one ragged sort of all buckets, merge passes for buckets longer
+ * than a chunk.  It synchronises the stream twice (bucket sizes, release of
+ * its temporaries); the reference synchronises per recursion level.
+ *
+ * cs_sort_hybrid replaces HybridSort / HybridSortV3 (cusort.hpp:2656-2754):
+ * merge below cs_hybrid_threshold(dtype) elements, partition above.  With a
+ * workspace of capacity >= n no memory is allocated.                        */
+typedef struct cs_workspace cs_workspace;
+int cs_workspace_create(cs_workspace** ws, int dtype, size_t capacity, int32_t blocks_per_cu, int32_t n_streams);
+int cs_workspace_reserve(cs_workspace* ws, size_t capacity);
+int cs_workspace_capacity(const cs_workspace* ws, size_t* capacity);
+int cs_workspace_destroy(cs_workspace* ws);
+int cs_sort_merge(int dtype, void* data, int64_t n, int32_t tile, int32_t merge_threads, void* ext_buf,
+                  size_t ext_elems, int stored_in_right_place, hipStream_t stream);
+int cs_sort_merge_ws(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
+int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32_t max_parts, uint32_t flags,
+                      hipStream_t stream);
+int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
+int64_t cs_hybrid_threshold(int dtype);
+
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel (16-byte
  * accesses when both pointers are 16-byte aligned).  The reference harness
  * restores every input with a device copy before each trial

@@ -1,0 +1,81 @@
+"""GPU parity of the large-array sorts (merge, partition, hybrid) against
+the CPU oracle (std::sort / __gnu_parallel::sort), including the full-size
+2^28 int32 and 2^27 uint64 configurations."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def run_large(cs, orc, rt, dtype, dist, n, method, seed=42, **kw):
+    host = orc.generate(dtype, dist, max(n, 1), n, seed)
+    buf = cs.from_numpy(host) if n else cs.device_buffer(dtype, 1)
+    if method == "merge":
+        cs.sort_merge(dtype, buf.ptr, n, **kw)
+    elif method == "merge_out":
+        out = cs.device_buffer(dtype, n)
+        cs.sort_merge(dtype, buf.ptr, n, ext=out.ptr, ext_elems=n, in_place=False)
+        buf = out
+    elif method == "merge_ws":
+        ws = cs.Workspace(dtype, n)
+        cs.sort_merge_ws(dtype, buf.ptr, n, ws)
+    elif method == "partition":
+        cs.sort_partition(dtype, buf.ptr, n, **kw)
+    elif method == "hybrid":
+        cs.sort_hybrid(dtype, buf.ptr, n, kw.get("ws"))
+    rt.synchronize()
+    got = cs.to_numpy(buf, dtype, n)
+    where = f"dtype={dtype} dist={orc.DIST_NAMES[dist]} n={n} method={method}"
+    if dtype in (cs.REC_I32_I32, cs.REC_I64_I32) or n <= (1 << 22):
+        rc, msg = orc.validate_segmented(dtype, host, got, np.array([0, n], dtype=np.int64))
+        assert rc == 0, f"{where}: {msg}"
+    else:
+        want = orc.sort_large(dtype, host)
+        assert got.tobytes() == want.tobytes(), where
+
+
+SIZES = [0, 1, 2, 33, 1000, 4096, 4097, 8192, 8193, 100003, (1 << 20) + 3, 3 << 20]
+DISTS = ["D_UNIFORM", "D_SORTED", "D_REVERSED", "D_CONSTANT", "D_DUP50", "D_EXTREMES6", "D_DISTINCT16",
+         "D_PRESORTED_RUNS"]
+
+
+@pytest.mark.parametrize("method", ["merge", "merge_out", "merge_ws", "partition", "hybrid"])
+@pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32"])
+def test_large_methods(cs, orc, gpu, dtype, method):
+    dt = getattr(cs, dtype)
+    for n in SIZES:
+        dists = DISTS if n in (100003, (1 << 20) + 3) else ["D_UNIFORM", "D_DUP50"]
+        for d in dists:
+            run_large(cs, orc, gpu, dt, getattr(orc, d), n, method, seed=n + 7)
+
+
+def test_partition_part_counts(cs, orc, gpu):
+    n = (1 << 21) + 5
+    for parts in (2, 3, 32, 1024, 2048):
+        run_large(cs, orc, gpu, cs.I32, orc.D_UNIFORM, n, "partition", max_parts=parts)
+    run_large(cs, orc, gpu, cs.U64, orc.D_DISTINCT16, n, "partition", max_parts=64)
+
+
+def test_merge_arguments(cs, gpu):
+    buf = cs.device_buffer(cs.I32, 16)
+    with pytest.raises(cs.CsError):
+        cs.sort_merge(cs.I32, buf.ptr, 16, in_place=False)          # out of place needs ext
+    with pytest.raises(cs.CsError):
+        cs.sort_merge(cs.I32, buf.ptr, 16, tile=100)                 # not a power of two
+    ws = cs.Workspace(cs.I32, 8)
+    with pytest.raises(cs.CsError):
+        cs.sort_merge_ws(cs.I32, buf.ptr, 16, ws)                    # workspace too small
+    ws.reserve(16)
+    assert ws.capacity() == 16
+    with pytest.raises(cs.CsError):
+        cs.sort_hybrid(cs.U64, buf.ptr, 2, ws)                       # wrong element type
+
+
+@pytest.mark.parametrize("dtype,n", [("I32", 1 << 28), ("U64", 1 << 27)])
+@pytest.mark.parametrize("dist", ["D_UNIFORM", "D_PRESORTED_RUNS"])
+def test_config4_full(cs, orc, gpu, dtype, n, dist):
+    dt = getattr(cs, dtype)
+    ws = cs.Workspace(dt, n)
+    run_large(cs, orc, gpu, dt, getattr(orc, dist), n, "hybrid", ws=ws)
+    run_large(cs, orc, gpu, dt, getattr(orc, dist), n, "merge")
+    run_large(cs, orc, gpu, dt, getattr(orc, dist), n, "partition")
