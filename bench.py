@@ -38,7 +38,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_i32",
                     choices=["batched_i32", "batched_rec32", "copy", "ragged_i32", "ragged_f32",
-                             "large_i32", "large_u64"])
+                             "large_i32", "large_u64", "serve"])
+    ap.add_argument("--rate", type=float, default=20000.0)
+    ap.add_argument("--seconds", type=float, default=10.0)
+    ap.add_argument("--budget-gb", type=float, default=32.0)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--large-method", default="hybrid", choices=["hybrid", "merge", "partition"])
     ap.add_argument("--large-log2", type=int, default=0, help="log2 of the array length (0: 28 for int32, 27 for uint64)")
     ap.add_argument("--large-dist", default="uniform", choices=["uniform", "presorted"])
@@ -299,6 +303,27 @@ def bench_large(args, dtype_name: str):
             "cpu_baseline": cpu, "validated": ok}
 
 
+def bench_serve(args):
+    """Serving loop: Poisson arrivals at --rate req/s for --seconds, sizes
+    log-uniform over 16..65536 elements, {int64 key, int32 payload} records,
+    device pool of --budget-gb GB (decimal), --slots in-flight batches.  The
+    open-loop generator is cuda-sort_amd/cusort_serve_bench (C++)."""
+    import subprocess
+    exe = os.path.join(ROOT, "cuda-sort_amd", "cusort_serve_bench")
+    cmd = [exe, "--rate", str(args.rate), "--seconds", str(args.seconds), "--budget-gb", str(args.budget_gb),
+           "--slots", str(args.slots)]
+    r = subprocess.run(cmd, capture_output=True, text=True, check=False)
+    if r.returncode != 0 and not r.stdout.strip():
+        raise RuntimeError(f"{exe} failed: {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"metric": METRIC, "value": res["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1,
+            "steps": res["requests"], "warmup": 0, "ms_per_step": res["mean_ms"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int64 key + int32 payload", "data": "synthetic",
+            "config": {"workload": f"serving loop, Poisson {args.rate:g} req/s for {args.seconds:g} s, sizes "
+                                   f"log-uniform 16..65536, {args.budget_gb:g} GB HBM budget, {args.slots} slots"},
+            "serve": res, "validated": res["failures"] == 0}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -310,6 +335,8 @@ def main():
         line = bench_batched(args, "i32")
     elif args.workload == "copy":
         line = bench_copy(args)
+    elif args.workload == "serve":
+        line = bench_serve(args)
     elif args.workload.startswith("large"):
         line = bench_large(args, args.workload.split("_")[1])
     elif args.workload.startswith("ragged"):

@@ -10,7 +10,7 @@
 
 #include "../../include/cusort_hip.h"
 #include "gen.hpp"
-#include "segmented.hpp"
+#include "ragged.hpp"
 
 namespace cs {
 
@@ -185,49 +185,16 @@ int make_plan_impl(cs_segplan** out, int dtype, const int64_t* h_offsets, int64_
     p->segments = segments;
     p->k = seg_kernels_for(dtype);
     p->use_graph = (flags & CS_SEG_NO_GRAPH) == 0;
-    const int maxb = p->k.max_logw;
-    // bucket the segments
-    std::vector<std::vector<int>> per(kMaxBuckets);
-    for (int64_t s = 0; s < segments; ++s) {
-        const int64_t len = h_offsets[s + 1] - h_offsets[s];
-        if (len < 0) { delete p; return CS_E_INVALID_VALUE; }
-        if (len > (int64_t(1) << maxb)) { delete p; return CS_E_CAPACITY; }
-        p->max_len = std::max(p->max_len, len);
-        const int b = bucket_of(len);
-        per[size_t(b)].push_back(int(s));
-    }
-    p->total = segments ? h_offsets[segments] - h_offsets[0] : 0;
     std::vector<int> list;
     std::vector<SegItem> items;
-    list.reserve(size_t(segments));
-    for (int b = maxb; b >= 0; --b) {
-        const auto& seg = per[size_t(b)];
-        p->bucket_segments[b] = int64_t(seg.size());
-        int v = force_variant ? force_variant[b] : tuned_variant(dtype, b);
-        if (v < 0 || !p->k.variant_ok(b, v)) v = 0;
-        if (!p->k.variant_ok(b, v)) v = 2;
-        p->variant[b] = v;
-        if (seg.empty()) continue;
-        const int spi = p->k.segments_per_item(b, v);
-        for (size_t i = 0; i < seg.size(); i += size_t(spi)) {
-            SegItem it;
-            it.bucket = b;
-            it.variant = v;
-            it.first = int(list.size() + i);
-            it.count = int(std::min(seg.size() - i, size_t(spi)));
-            items.push_back(it);
-        }
-        list.insert(list.end(), seg.begin(), seg.end());
-    }
+    const int rc0 = seg_build_items(dtype, h_offsets, segments, force_variant, list, items, &p->max_len, p->variant,
+                                    p->bucket_segments);
+    if (rc0 != CS_OK) { delete p; return rc0; }
+    p->total = segments ? h_offsets[segments] - h_offsets[0] : 0;
     p->nitems = int(items.size());
-    int dev = 0, cus = 1, per_cu = 1;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->k.persistent_fn, kSegBlock, 0);
-    if (e != hipSuccess) { free_plan(p); return int(e); }
-    p->grid = std::max(1, std::min(cus * std::max(per_cu, 1), std::max(p->nitems, 1)));
-    e = hipMalloc(&p->d_list, std::max<size_t>(list.size(), 1) * sizeof(int));
+    p->grid = seg_persistent_grid(dtype, p->nitems);
+    if (p->grid <= 0) { free_plan(p); return CS_E_NO_DEVICE; }
+    hipError_t e = hipMalloc(&p->d_list, std::max<size_t>(list.size(), 1) * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&p->d_items, std::max<size_t>(items.size(), 1) * sizeof(SegItem));
     if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
     if (e == hipSuccess && !list.empty())
@@ -259,6 +226,112 @@ int run_plan(cs_segplan* p, const void* in, void* out, const int64_t* d_offsets,
 }
 
 }  // namespace
+
+namespace cs {
+
+int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* force_variant, std::vector<int>& list,
+                    std::vector<SegItem>& items, int64_t* max_len, int* variants, int64_t* bucket_segments) {
+    const SegKernels k = seg_kernels_for(dtype);
+    const int maxb = k.max_logw;
+    std::vector<std::vector<int>> per(kMaxBuckets);
+    int64_t mx = 0;
+    for (int64_t s = 0; s < segs; ++s) {
+        const int64_t len = h_off[s + 1] - h_off[s];
+        if (len < 0) return CS_E_INVALID_VALUE;
+        if (len > (int64_t(1) << maxb)) return CS_E_CAPACITY;
+        mx = std::max(mx, len);
+        per[size_t(bucket_of(len))].push_back(int(s));
+    }
+    if (max_len) *max_len = mx;
+    list.clear();
+    items.clear();
+    list.reserve(size_t(segs));
+    for (int b = maxb; b >= 0; --b) {
+        const auto& seg = per[size_t(b)];
+        if (bucket_segments) bucket_segments[b] = int64_t(seg.size());
+        int v = force_variant ? force_variant[b] : tuned_variant(dtype, b);
+        if (v < 0 || !k.variant_ok(b, v)) v = 0;
+        if (!k.variant_ok(b, v)) v = 2;
+        if (variants) variants[b] = v;
+        if (seg.empty()) continue;
+        const int spi = k.segments_per_item(b, v);
+        for (size_t i = 0; i < seg.size(); i += size_t(spi)) {
+            SegItem it;
+            it.bucket = b;
+            it.variant = v;
+            it.first = int(list.size() + i);
+            it.count = int(std::min(seg.size() - i, size_t(spi)));
+            items.push_back(it);
+        }
+        list.insert(list.end(), seg.begin(), seg.end());
+    }
+    return CS_OK;
+}
+
+int seg_persistent_grid(int dtype, int nitems) {
+    static std::mutex mu;
+    static int cache[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!cache[dtype]) {
+        int dev = 0, cus = 1, per_cu = 1;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, seg_kernels_for(dtype).persistent_fn,
+                                                              kSegBlock, 0);
+        if (e != hipSuccess) { (void)hipGetLastError(); return 0; }
+        cache[dtype] = std::max(1, cus * std::max(per_cu, 1));
+    }
+    return std::max(1, std::min(cache[dtype], std::max(nitems, 1)));
+}
+
+int ragged_prepare(int dtype, const int64_t* h_off, int64_t segs, RaggedPrep* p) {
+    const LargeKernels lk = large_kernels_for(dtype);
+    const int64_t chunk = lk.chunk;
+    p->chunk_off.assign(1, h_off[0]);
+    p->tiles.clear();
+    p->passes = 0;
+    for (int64_t s = 0; s < segs; ++s) {
+        const int64_t b0 = h_off[s], len = h_off[s + 1] - b0;
+        if (len < 0) return CS_E_INVALID_VALUE;
+        if (len == 0) { p->chunk_off.push_back(b0); continue; }
+        for (int64_t c = 0; c < len; c += chunk) p->chunk_off.push_back(b0 + std::min(len, c + chunk));
+        if (len > chunk) {
+            int q = 0;
+            for (int64_t run = chunk; run < len; run *= 2) ++q;
+            p->passes = std::max(p->passes, q);
+            for (int64_t d0 = 0; d0 < len; d0 += lk.tile) p->tiles.push_back(MergeTile{b0, len, d0});
+        }
+    }
+    const int rc = seg_build_items(dtype, p->chunk_off.data(), int64_t(p->chunk_off.size()) - 1, nullptr, p->list,
+                                   p->items, nullptr, nullptr, nullptr);
+    if (rc != CS_OK) return rc;
+    p->grid = seg_persistent_grid(dtype, int(p->items.size()));
+    return p->grid > 0 ? CS_OK : CS_E_NO_DEVICE;
+}
+
+hipError_t ragged_launch(int dtype, const void* in, void* out, void* scratch, const int64_t* d_chunk_off,
+                         const int* d_list, const SegItem* d_items, int nitems, int* d_counter, int grid,
+                         const MergeTile* d_tiles, int ntiles, int passes, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counter);
+    hipError_t e = hipGetLastError();
+    const SegKernels k = seg_kernels_for(dtype);
+    if (e == hipSuccess) e = k.persistent(in, out, d_chunk_off, d_list, d_items, nitems, d_counter, grid, s);
+    if (e != hipSuccess || ntiles <= 0 || passes <= 0) return e;
+    const PartKernels pk = part_kernels_for(dtype);
+    const int64_t chunk = large_kernels_for(dtype).chunk;
+    void* a = out;
+    void* b = scratch;
+    for (int q = 0; e == hipSuccess && q < passes; ++q) {
+        e = pk.seg_merge(a, b, d_tiles, ntiles, chunk << q, 0, s);
+        std::swap(a, b);
+    }
+    if (e == hipSuccess && a != out) e = pk.seg_merge(a, out, d_tiles, ntiles, 0, 1, s);
+    return e;
+}
+
+}  // namespace cs
 
 extern "C" {
 

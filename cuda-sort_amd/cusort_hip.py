@@ -111,6 +111,25 @@ def lib():
         L.cs_sort_hybrid.argtypes = [ctypes.c_int, vp, i64, vp, vp]
         L.cs_hybrid_threshold.restype = i64
         L.cs_hybrid_threshold.argtypes = [ctypes.c_int]
+        L.cs_server_now.restype = ctypes.c_double
+        L.cs_server_create.restype = ctypes.c_int
+        L.cs_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, sz, i32, i64, i32]
+        L.cs_server_submit.restype = ctypes.c_int
+        L.cs_server_submit.argtypes = [vp, vp, vp, i64, ctypes.c_double, ctypes.POINTER(u64)]
+        L.cs_server_poll.restype = ctypes.c_int
+        L.cs_server_poll.argtypes = [vp, u64, P32, P32]
+        L.cs_server_wait.restype = ctypes.c_int
+        L.cs_server_wait.argtypes = [vp, u64, ctypes.c_double]
+        L.cs_server_latency.restype = ctypes.c_int
+        L.cs_server_latency.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.cs_server_release.restype = ctypes.c_int
+        L.cs_server_release.argtypes = [vp, u64]
+        L.cs_server_drain.restype = ctypes.c_int
+        L.cs_server_drain.argtypes = [vp]
+        L.cs_server_get_stats.restype = ctypes.c_int
+        L.cs_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.cs_server_destroy.restype = ctypes.c_int
+        L.cs_server_destroy.argtypes = [vp]
         L.cs_device_copy.restype = ctypes.c_int
         L.cs_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.cs_generate.restype = ctypes.c_int
@@ -169,6 +188,84 @@ def device_copy(dst: int, src: int, nbytes: int, stream: int = 0) -> None:
 
 
 SEG_NO_GRAPH = 1
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_int64), ("rejected", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("keys", ctypes.c_int64), ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double),
+                ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("keys_per_s", ctypes.c_double),
+                ("mean_batch_requests", ctypes.c_double), ("peak_pool_bytes", ctypes.c_int64),
+                ("max_inflight_batches", ctypes.c_int64)]
+
+
+class Server:
+    """cs_server: continuous batching of host sort requests."""
+
+    def __init__(self, dtype: int, budget_bytes: int, slots: int = 4, max_batch_elems: int = 0,
+                 max_batch_requests: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().cs_server_create(ctypes.byref(h), dtype, budget_bytes, slots, max_batch_elems,
+                                     max_batch_requests), "cs_server_create")
+        self.handle = h.value
+        self.dtype = dtype
+        self._keep = {}
+
+    def submit(self, arr, out=None, arrival_s: float = -1.0) -> int:
+        """Submit a host numpy array; returns the ticket.  The result lands in
+        `out` (a new array by default), available from result(ticket)."""
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        o = np.empty_like(a) if out is None else out
+        t = ctypes.c_uint64(0)
+        check(lib().cs_server_submit(self.handle, a.ctypes.data if a.size else None,
+                                     o.ctypes.data if o.size else None, a.size, arrival_s, ctypes.byref(t)),
+              "cs_server_submit")
+        self._keep[t.value] = (a, o)
+        return int(t.value)
+
+    def submit_status(self, arr) -> int:
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        o = np.empty_like(a)
+        t = ctypes.c_uint64(0)
+        return int(lib().cs_server_submit(self.handle, a.ctypes.data, o.ctypes.data, a.size, -1.0, ctypes.byref(t)))
+
+    def wait(self, ticket: int, timeout_s: float = -1.0) -> int:
+        return int(lib().cs_server_wait(self.handle, ticket, timeout_s))
+
+    def result(self, ticket: int):
+        rc = self.wait(ticket)
+        check(rc, "server request")
+        return self._keep[ticket][1]
+
+    def latency(self, ticket: int) -> float:
+        a, d = ctypes.c_double(0), ctypes.c_double(0)
+        check(lib().cs_server_latency(self.handle, ticket, ctypes.byref(a), ctypes.byref(d)), "cs_server_latency")
+        return d.value - a.value
+
+    def release(self, ticket: int) -> None:
+        check(lib().cs_server_release(self.handle, ticket), "cs_server_release")
+        self._keep.pop(ticket, None)
+
+    def drain(self) -> None:
+        check(lib().cs_server_drain(self.handle), "cs_server_drain")
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().cs_server_get_stats(self.handle, ctypes.byref(st)), "cs_server_get_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.cs_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+def server_now() -> float:
+    return float(lib().cs_server_now())
 
 
 class Workspace:

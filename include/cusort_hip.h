@@ -202,6 +202,50 @@ int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32
 int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
 int64_t cs_hybrid_threshold(int dtype);
 
+/* ----------------------------------------------------------------- serving
+ * Continuous batching of sort requests on one GPU.  No reference
+ * counterpart.  A request is a host array of n elements (in) and a host
+ * output array (out, may equal in); it completes when out holds the sorted
+ * array.  Requests are queued in order; whenever one of `slots` in-flight
+ * slots is free the scheduler packs pending requests into one ragged batch
+ * (up to max_batch_elems elements and max_batch_requests requests; 0 picks
+ * 4M and 4096) provided the device pool, sized to the HBM admission budget
+ * `budget_bytes`, can hold it (a request longer than one tile also needs
+ * merge scratch of its own size); otherwise it waits for a batch to complete.
+ * Each batch is copied in, sorted (one persistent launch plus merge passes
+ * for long requests) and copied out on its slot's stream, so batches
+ * overlap on the device.  Submission fails with CS_E_BUDGET for a request
+ * that could never fit.  Latency = arrival (the caller's timestamp from
+ * cs_server_now(), or the submission time when arrival_s < 0) to the moment
+ * the result is in `out`.  The server keeps every request until released. */
+typedef struct cs_server cs_server;
+typedef struct cs_server_stats {
+    int64_t completed;          /* requests completed successfully */
+    int64_t rejected;           /* submissions refused (CS_E_BUDGET) */
+    int64_t batches;            /* batches launched */
+    int64_t keys;               /* elements sorted */
+    double p50_s, p99_s, max_s, mean_s;   /* latency percentiles, seconds */
+    double keys_per_s;          /* keys / (last completion - first arrival) */
+    double mean_batch_requests; /* average requests per batch */
+    int64_t peak_pool_bytes;    /* high-water mark of the device pool */
+    int64_t max_inflight_batches;
+} cs_server_stats;
+double cs_server_now(void);
+int cs_server_create(cs_server** s, int dtype, size_t budget_bytes, int32_t slots, int64_t max_batch_elems,
+                     int32_t max_batch_requests);
+int cs_server_submit(cs_server* s, const void* host_in, void* host_out, int64_t n, double arrival_s,
+                     uint64_t* ticket);
+/* done = 1 once finished; status = the request's result code. */
+int cs_server_poll(cs_server* s, uint64_t ticket, int32_t* done, int32_t* status);
+/* Blocks until the request finishes (timeout_s < 0: no limit); returns its
+ * status, or CS_E_AGAIN on time-out. */
+int cs_server_wait(cs_server* s, uint64_t ticket, double timeout_s);
+int cs_server_latency(cs_server* s, uint64_t ticket, double* arrival_s, double* done_s);
+int cs_server_release(cs_server* s, uint64_t ticket);
+int cs_server_drain(cs_server* s);
+int cs_server_get_stats(cs_server* s, cs_server_stats* stats);
+int cs_server_destroy(cs_server* s);
+
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel (16-byte
  * accesses when both pointers are 16-byte aligned).  The reference harness
  * restores every input with a device copy before each trial

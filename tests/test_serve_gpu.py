@@ -1,0 +1,106 @@
+"""GPU tests of the serving layer (cs_server_*): every result equals the
+oracle's std::sort, a request gives the same key sequence alone and in a
+concurrent mix (records: same keys, payloads a valid permutation; tie order
+is free), the admission budget bounds the pool and back-pressures, an
+impossible request is refused, and batches overlap on the device."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_request(orc, dtype, n, seed, dist=None):
+    """One request: keys from the oracle's generator, payload = local index."""
+    dist = orc.D_UNIFORM if dist is None else dist
+    a = orc.generate(dtype, dist, max(n, 1), n, seed)
+    if dtype in (orc.REC_I32_I32, orc.REC_I64_I32):
+        a["payload"] = np.arange(n, dtype=np.int32)
+    return a
+
+
+def check_one(orc, dtype, a, out):
+    rc, msg = orc.validate_segmented(dtype, a, out, np.array([0, a.size], dtype=np.int64))
+    assert rc == 0, msg
+
+
+SIZES = [0, 1, 2, 16, 100, 4095, 4096, 4097, 8192, 20000, 65536]
+
+
+def sizes_mix(rng, count):
+    s = np.floor(np.exp(rng.uniform(np.log(16), np.log(65537), count))).astype(np.int64)
+    return list(np.clip(s, 16, 65536)) + SIZES
+
+
+@pytest.mark.parametrize("dtype", ["REC_I64_I32", "I32", "F32", "REC_I32_I32"])
+def test_results_match_oracle(cs, orc, gpu, dtype):
+    dt = getattr(cs, dtype)
+    rng = np.random.default_rng(3 + dt)
+    srv = cs.Server(dt, 1 << 30, slots=4)
+    reqs = []
+    dists = [orc.D_UNIFORM, orc.D_DUP50, orc.D_EXTREMES6, orc.D_SORTED]
+    for i, n in enumerate(sizes_mix(rng, 150)):
+        a = make_request(orc, dt, int(n), 1000 + i, dists[i % 4])
+        reqs.append((srv.submit(a), a))
+    for t, a in reqs:
+        check_one(orc, dt, a, srv.result(t))
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["rejected"] == 0
+    srv.close()
+
+
+def test_alone_vs_mix(cs, orc, gpu):
+    dt = cs.REC_I64_I32
+    rng = np.random.default_rng(9)
+    sizes = sizes_mix(rng, 120)
+    reqs = [make_request(orc, dt, int(n), 77 + i, orc.D_DUP50 if i % 2 else orc.D_UNIFORM) for i, n in enumerate(sizes)]
+    # in the mix: everything submitted at once, batches overlapping
+    mix = cs.Server(dt, 1 << 30, slots=4, max_batch_requests=8)
+    tickets = [mix.submit(a) for a in reqs]
+    mixed = [mix.result(t) for t in tickets]
+    st = mix.stats()
+    mix.close()
+    # alone: one request at a time on a fresh server
+    solo = cs.Server(dt, 1 << 30, slots=1)
+    for a, m in zip(reqs, mixed):
+        t = solo.submit(a)
+        alone = solo.result(t)
+        check_one(orc, dt, a, alone)
+        check_one(orc, dt, a, m)
+        assert alone["key"].tobytes() == m["key"].tobytes()
+    solo.close()
+    assert st["batches"] < len(reqs)            # requests were coalesced
+    assert st["max_inflight_batches"] >= 2      # batches overlapped on the device
+
+
+def test_budget_backpressure_and_rejection(cs, orc, gpu):
+    dt = cs.REC_I64_I32
+    budget = 4 << 20                             # 4 MiB: a few requests at a time
+    srv = cs.Server(dt, budget, slots=4)
+    rng = np.random.default_rng(1)
+    reqs = [make_request(orc, dt, int(n), i) for i, n in enumerate(rng.integers(1000, 60000, 60))]
+    tickets = [srv.submit(a) for a in reqs]
+    for t, a in zip(tickets, reqs):
+        check_one(orc, dt, a, srv.result(t))
+    st = srv.stats()
+    assert 0 < st["peak_pool_bytes"] <= budget
+    assert st["completed"] == len(reqs)
+    # a long request needs twice its size (merge scratch): 150k records = 4.8 MB > budget
+    big = make_request(orc, dt, 150000, 5)
+    assert srv.submit_status(big) == cs.E_BUDGET
+    assert srv.stats()["rejected"] == 1
+    srv.close()
+
+
+def test_latency_bookkeeping(cs, orc, gpu):
+    srv = cs.Server(cs.I32, 1 << 28, slots=2)
+    a = make_request(orc, cs.I32, 5000, 3)
+    t0 = cs.server_now()
+    t = srv.submit(a, arrival_s=t0)
+    srv.result(t)
+    lat = srv.latency(t)
+    assert 0 < lat < 5.0
+    srv.release(t)
+    srv.drain()
+    st = srv.stats()
+    assert st["p50_s"] > 0 and st["p99_s"] >= st["p50_s"] and st["keys"] == 5000
+    srv.close()
