@@ -1,0 +1,111 @@
+// Uses the C++ drop-in header (cuda-sort_amd/cusort.hpp) the way the
+// reference's callers use cusort.hpp (test.cu:58-61, 471, 525, 578; bench.cu
+// 453-534, 660-666): device buffers, the template entry points, host
+// std::sort as the check.  Exit status 0 when every case matches.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "../../cuda-sort_amd/cusort.hpp"
+
+using cusort_hip::IntAndData;
+
+static int failures = 0;
+
+template <class T> bool key_less(const T& a, const T& b) { return a < b; }
+
+template <class T>
+void expect_sorted_equal(std::vector<T> in, const std::vector<T>& got, int n_per, const char* what) {
+    for (size_t a = 0; a < in.size(); a += size_t(n_per))
+        std::sort(in.begin() + long(a), in.begin() + long(std::min(in.size(), a + size_t(n_per))), key_less<T>);
+    for (size_t i = 0; i < in.size(); ++i) {
+        const bool same = !(in[i] < got[i]) && !(got[i] < in[i]);
+        if (!same) {
+            std::printf("FAIL %s at %zu\n", what, i);
+            ++failures;
+            return;
+        }
+    }
+    std::printf("PASS %s\n", what);
+}
+
+template <class T> T* upload(const std::vector<T>& h) {
+    T* d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(h.size(), 1) * sizeof(T)) != hipSuccess) std::exit(2);
+    if (!h.empty()) (void)hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    return d;
+}
+template <class T> std::vector<T> download(const T* d, size_t n) {
+    std::vector<T> h(n);
+    (void)hipDeviceSynchronize();
+    if (n) (void)hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost);
+    return h;
+}
+
+int main() {
+    std::mt19937 rng(1234);
+    // batched: one array per workgroup, packed (bench.cu:355-393)
+    for (int n : {7, 32, 100, 1024, 4096}) {
+        const int arrays = 37;
+        std::vector<int> h(static_cast<size_t>(n) * arrays);
+        for (auto& v : h) v = int(rng());
+        int* d = upload(h);
+        if (n <= 32) cusort_hip::Sort_1Warp_SM_le32_batched<int>(d, n, arrays);
+        else cusort_hip::Sort_1Block_batched<int, 3, true, false, false>(d, n, arrays, 64);
+        expect_sorted_equal(h, download(d, h.size()), n, "batched v3/warp int");
+        (void)hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (n == 4096) {   // Working must be max(32, pow2(n)) (bench.cu:497-521)
+            cusort_hip::Sort_1Block_v6_register_batched<int, 4096, 256>(d, n, arrays);
+            expect_sorted_equal(h, download(d, h.size()), n, "batched v6 int");
+        }
+        (void)hipFree(d);
+    }
+    {
+        const int n = 1000, arrays = 50;
+        std::vector<IntAndData> h(static_cast<size_t>(n) * arrays);
+        for (size_t i = 0; i < h.size(); ++i) h[i] = IntAndData{int(rng() % 100), int(i)};
+        IntAndData* d = upload(h);
+        cusort_hip::Sort_1Block_v5_static_batched<IntAndData, 1024, 128>(d, n, arrays);
+        expect_sorted_equal(h, download(d, h.size()), n, "batched v5 IntAndData");
+        (void)hipFree(d);
+    }
+    // large arrays (test.cu:444-602 sizes)
+    for (int n : {33, 5000, 100000, 1 << 20}) {
+        std::vector<int> h(static_cast<size_t>(n));
+        for (auto& v : h) v = int(rng() % 1000000);
+        int* d = upload(h);
+        cusort_hip::SortMerge<int>(d, n);
+        expect_sorted_equal(h, download(d, h.size()), n, "SortMerge int");
+        (void)hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+        cusort_hip::Sort<int>(d, n);
+        expect_sorted_equal(h, download(d, h.size()), n, "Sort int");
+        (void)hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+        cusort_hip::CuSortWorkspace<int> ws{static_cast<size_t>(n)};
+        cusort_hip::HybridSort<int>(d, n, ws);
+        expect_sorted_equal(h, download(d, h.size()), n, "HybridSort int (workspace)");
+        (void)hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+        cusort_hip::HybridSortV3<int>(d, n);
+        expect_sorted_equal(h, download(d, h.size()), n, "HybridSortV3 int");
+        int* out = upload(std::vector<int>(size_t(n)));
+        (void)hipMemcpy(d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+        cusort_hip::SortMerge<int>(d, n, 0, nullptr, out, size_t(n), false);
+        expect_sorted_equal(h, download(out, h.size()), n, "SortMerge out of place");
+        (void)hipFree(out);
+        (void)hipFree(d);
+    }
+    {
+        const int n = 300000;
+        std::vector<unsigned long long> h(static_cast<size_t>(n));
+        for (auto& v : h) v = (uint64_t(rng()) << 32) | rng();
+        unsigned long long* d = upload(h);
+        cusort_hip::CuSortWorkspace<unsigned long long> ws{static_cast<size_t>(n)};
+        cusort_hip::HybridSortPartitionedV3<unsigned long long>(d, n, ws);
+        expect_sorted_equal(h, download(d, h.size()), n, "HybridSortPartitionedV3 u64");
+        (void)hipFree(d);
+    }
+    std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
+    return failures ? 1 : 0;
+}
