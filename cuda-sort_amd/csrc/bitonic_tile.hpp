@@ -232,15 +232,20 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
 
     Tile<E, LOGW, LOGE> tile;
     // 16-byte accesses need 16-byte aligned array starts (n*sizeof % 16 == 0).
-    const bool vec_in = ((uintptr_t)src & 15) == 0 && VS == V && V > 1;
-    const bool vec_out = ((uintptr_t)dst & 15) == 0 && VS == V && V > 1;
+    const bool vec_in = ((uintptr_t)src & 15) == 0 && VS == V;
+    const bool vec_out = ((uintptr_t)dst & 15) == 0 && VS == V;
 
-    // ---- load, lane-striped
-    sfor<0, NE / VS>([&](auto q_) {
-        constexpr int q = decltype(q_)::value;
-        const int g = (q * T + t) * VS;
-        if (vec_in && g + VS <= valid) {
-            const Vec v = *reinterpret_cast<const Vec*>(src + g);
+    // ---- load, lane-striped.  When every group of this lane is in range
+    // (all lanes of a full array), the loads form one branch-free block, all
+    // in flight together; otherwise each group is bounds checked.  Stores
+    // are split the same way.
+    constexpr int NG = NE / VS;
+    const bool full_in = vec_in && ((NG - 1) * T + t) * VS + VS <= valid;
+    const bool full_out = vec_out && ((NG - 1) * T + t) * VS + VS <= valid;
+    if (full_in) {
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            const Vec v = *reinterpret_cast<const Vec*>(src + (q * T + t) * VS);
             Storage tmp[V];
             __builtin_memcpy(tmp, &v, 16);
             sfor<0, VS>([&](auto j_) {
@@ -248,20 +253,35 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
                 tile.k[q * VS + j] = E::key(tmp[j]);
                 tile.p[q * VS + j] = E::payload(tmp[j]);
             });
-        } else {
-            sfor<0, VS>([&](auto j_) {
-                constexpr int j = decltype(j_)::value;
-                if (g + j < valid) {
-                    const Storage s = src[g + j];
-                    tile.k[q * VS + j] = E::key(s);
-                    tile.p[q * VS + j] = E::payload(s);
-                } else {
-                    tile.k[q * VS + j] = E::kPad;
-                    tile.p[q * VS + j] = INT32_MAX;
-                }
-            });
-        }
-    });
+        });
+    } else {
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            const int g = (q * T + t) * VS;
+            if (vec_in && g + VS <= valid) {
+                const Vec v = *reinterpret_cast<const Vec*>(src + g);
+                Storage tmp[V];
+                __builtin_memcpy(tmp, &v, 16);
+                sfor<0, VS>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    tile.k[q * VS + j] = E::key(tmp[j]);
+                    tile.p[q * VS + j] = E::payload(tmp[j]);
+                });
+            } else {
+                sfor<0, VS>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    if (g + j < valid) {
+                        const Storage s = src[g + j];
+                        tile.k[q * VS + j] = E::key(s);
+                        tile.p[q * VS + j] = E::payload(s);
+                    } else {
+                        tile.k[q * VS + j] = E::kPad;
+                        tile.p[q * VS + j] = INT32_MAX;
+                    }
+                });
+            }
+        });
+    }
     if constexpr (E::kPayload && C::NTRANS > 0) tile.template io_transpose<VS, true>(t, lk, lp, kBufFirst);
 
     // ---- network
@@ -269,25 +289,33 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
 
     // ---- store, lane-striped
     if constexpr (C::NTRANS > 0) tile.template io_transpose<VS, false>(t, lk, lp, kBufLast);
-    sfor<0, NE / VS>([&](auto q_) {
+    auto store_group = [&](auto q_) {
         constexpr int q = decltype(q_)::value;
-        const int g = (q * T + t) * VS;
-        if (vec_out && g + VS <= valid) {
-            Storage tmp[V];
-            sfor<0, VS>([&](auto j_) {
-                constexpr int j = decltype(j_)::value;
-                tmp[j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
-            });
-            Vec v;
-            __builtin_memcpy(&v, tmp, 16);
-            *reinterpret_cast<Vec*>(dst + g) = v;
-        } else {
-            sfor<0, VS>([&](auto j_) {
-                constexpr int j = decltype(j_)::value;
-                if (g + j < valid) dst[g + j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
-            });
-        }
-    });
+        Storage tmp[V];
+        sfor<0, VS>([&](auto j_) {
+            constexpr int j = decltype(j_)::value;
+            tmp[j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
+        });
+        Vec v;
+        __builtin_memcpy(&v, tmp, 16);
+        *reinterpret_cast<Vec*>(dst + (q * T + t) * VS) = v;
+    };
+    if (full_out) {
+        sfor<0, NG>(store_group);
+    } else {
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            const int g = (q * T + t) * VS;
+            if (vec_out && g + VS <= valid) {
+                store_group(q_);
+            } else {
+                sfor<0, VS>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    if (g + j < valid) dst[g + j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
+                });
+            }
+        });
+    }
 }
 
 // Batched launch: array a = in + a*n, APB arrays per workgroup.
