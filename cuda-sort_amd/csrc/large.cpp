@@ -61,14 +61,21 @@ int merge_sort(int dtype, void* data, int64_t n, void* final, void* scratch, hip
     }
     int passes = 0;
     for (int64_t run = k.chunk; run < n; run *= 2) ++passes;
+    // Start co-rank of every merge tile (one int64 each), stream ordered.
+    const int64_t tiles = (n + k.tile - 1) / k.tile;
+    int64_t* cor = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&cor), size_t(tiles) * sizeof(int64_t), stream);
+    if (e != hipSuccess) return int(e);
     // The chunk sort lands where an even number of ping-pong passes ends in `final`.
     void* a = (passes % 2 == 0) ? final : scratch;
     void* b = (a == final) ? scratch : final;
-    hipError_t e = k.chunk_sort(data, a, n, stream);
+    e = k.chunk_sort(data, a, n, stream);
     for (int64_t run = k.chunk; e == hipSuccess && run < n; run *= 2) {
-        e = k.merge_pass(a, b, n, run, stream);
+        e = k.merge_pass(a, b, n, run, cor, stream);
         std::swap(a, b);
     }
+    const hipError_t fe = hipFreeAsync(cor, stream);
+    if (e == hipSuccess) e = fe;
     return e == hipSuccess ? CS_OK : int(e);
 }
 

@@ -117,14 +117,30 @@ __global__ void k_scan_apply(int64_t* v, int64_t n, const int64_t* sums);
 __global__ void k_bucket_totals(const int64_t* offsets, int nb, int blocks, int64_t n, int64_t* bucket_off);
 
 // Merge pass over a list of oversized buckets: item = (bucket base, bucket
-// length, first output position of this tile inside the bucket).  With
-// copy_only the tile is copied unchanged (equality buckets, final
+// length, first output position of this tile inside the bucket), plus the
+// co-rank of the tile's first output, which k_seg_merge_corank fills in
+// before every pass.  The tiles of a bucket are consecutive items, so a
+// tile that does not end its run pair finds its end co-rank in the next item.
+// With copy_only the tile is copied unchanged (equality buckets, final
 // placement after an odd number of passes).
 struct MergeTile {
     int64_t base;
     int64_t len;
     int64_t d0;
+    int64_t a0 = 0;   // co-rank of the tile's first output
 };
+
+template <int DT>
+__global__ void __launch_bounds__(kCorankThreads)
+k_seg_merge_corank(const typename Elem<DT>::Storage* src, MergeTile* items, int nitems, int64_t run) {
+    using E = Elem<DT>;
+    const int q = int(blockIdx.x) * kCorankThreads + int(threadIdx.x);
+    if (q >= nitems) return;
+    const MergeTile it = items[q];
+    const MergeGeom g = merge_geom<merge_tile<DT>()>(it.len, run, it.d0);
+    const typename E::Storage* A = src + it.base + g.base;
+    items[q].a0 = thread_corank<E>(A, g.na, A + g.na, g.nb, g.d0);
+}
 
 template <int DT>
 __global__ void __launch_bounds__(kMergeThreads)
@@ -140,7 +156,9 @@ k_seg_merge_pass(const typename Elem<DT>::Storage* src, typename Elem<DT>::Stora
         for (int64_t i = it.d0 + tid; i < end; i += kMergeThreads) dst[it.base + i] = src[it.base + i];
         return;
     }
-    merge_tile_body<DT>(src + it.base, dst + it.base, it.len, run, it.d0);
+    const MergeGeom g = merge_geom<TILE>(it.len, run, it.d0);
+    merge_tile_body<DT>(src + it.base, dst + it.base, it.len, run, it.d0, it.a0,
+                        merge_end_corank(g, &items[blockIdx.x + 1].a0));
 }
 
 struct PartKernels {
@@ -150,7 +168,9 @@ struct PartKernels {
                         int64_t* counts, hipStream_t s) = nullptr;
     hipError_t (*scatter)(const void* src, int64_t n, const void* splitters, int ns, int64_t per_block, int blocks,
                           const int64_t* offsets, void* dst, hipStream_t s) = nullptr;
-    hipError_t (*seg_merge)(const void* src, void* dst, const MergeTile* items, int nitems, int64_t run,
+    // Co-rank search + merge pass (or a plain copy with copy_only); writes
+    // the co-ranks into items.
+    hipError_t (*seg_merge)(const void* src, void* dst, MergeTile* items, int nitems, int64_t run,
                             int copy_only, hipStream_t s) = nullptr;
     int key_size = 0;
 };
@@ -185,9 +205,16 @@ PartKernels part_kernels() {
                            static_cast<S*>(dst));
         return hipGetLastError();
     };
-    k.seg_merge = [](const void* src, void* dst, const MergeTile* items, int nitems, int64_t run, int copy_only,
+    k.seg_merge = [](const void* src, void* dst, MergeTile* items, int nitems, int64_t run, int copy_only,
                      hipStream_t s) {
         if (nitems <= 0) return hipSuccess;
+        if (!copy_only) {
+            const int64_t cblocks = (int64_t(nitems) + kCorankThreads - 1) / kCorankThreads;
+            hipLaunchKernelGGL(k_seg_merge_corank<DT>, dim3(unsigned(cblocks)), dim3(kCorankThreads), 0, s,
+                               static_cast<const S*>(src), items, nitems, run);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
         hipLaunchKernelGGL(k_seg_merge_pass<DT>, dim3(unsigned(nitems)), dim3(kMergeThreads), 0, s,
                            static_cast<const S*>(src), static_cast<S*>(dst), items, run, copy_only);
         return hipGetLastError();

@@ -28,7 +28,7 @@ int ragged_prepare(int dtype, const int64_t* h_off, int64_t segs, RaggedPrep* p)
 // out and scratch (same layout as out; only touched when tiles exist).
 hipError_t ragged_launch(int dtype, const void* in, void* out, void* scratch, const int64_t* d_chunk_off,
                          const int* d_list, const SegItem* d_items, int nitems, int* d_counter, int grid,
-                         const MergeTile* d_tiles, int ntiles, int passes, hipStream_t s);
+                         MergeTile* d_tiles, int ntiles, int passes, hipStream_t s);
 
 // Bucketing shared with cs_segplan: list and items for CSR offsets whose
 // segments all fit one tile; force_variant (per bucket) may be null.

@@ -312,7 +312,7 @@ int ragged_prepare(int dtype, const int64_t* h_off, int64_t segs, RaggedPrep* p)
 
 hipError_t ragged_launch(int dtype, const void* in, void* out, void* scratch, const int64_t* d_chunk_off,
                          const int* d_list, const SegItem* d_items, int nitems, int* d_counter, int grid,
-                         const MergeTile* d_tiles, int ntiles, int passes, hipStream_t s) {
+                         MergeTile* d_tiles, int ntiles, int passes, hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counter);
     hipError_t e = hipGetLastError();
