@@ -5,8 +5,13 @@
 // bucket b = ceil(log2(len)) (W = 2^b) and the segments of a bucket are cut
 // into work items of as many segments as one workgroup of 256 threads sorts
 // at once with that bucket's tiling.  Items are listed largest bucket
-// first.  One launch of k_seg_persistent (a grid of a few workgroups per
-// CU) pulls items from a global counter until the list is exhausted; each
+// first.  Two persistent kernels share the list: one for the buckets whose
+// register tiles span several waves (and their LDS-resident variant), one
+// for the buckets whose tiles fit one wave.  Each is compiled for its own
+// buckets only, so its register and LDS footprint (and with it the number
+// of workgroups per CU) is set by those buckets and not by the largest
+// tile.  Each launch (a grid of a few workgroups per CU) pulls items of its
+// range from its own global counter until the range is exhausted; each
 // item runs the tile sort of its bucket with the variant the tuner chose:
 //   variant 0: register tile, 32 keys per lane (16 for 16-byte records)
 //   variant 1: register tile, 16 keys per lane
@@ -68,11 +73,18 @@ template <class E> constexpr int tile_lds_words(int logw, int loge) {
     return make_plan(logw, loge).ntrans ? (T > 64 ? 2 : 1) * apb * region : 0;
 }
 
-// LDS words (keys; payload uses the same count of int32) the persistent
-// kernel needs: the maximum over all bucket variants.
-template <int DT> constexpr int seg_lds_words() {
-    int m = 1 << seg_max_logw<DT>();    // LDS-resident variant: one segment of W
-    for (int b = 0; b <= seg_max_logw<DT>(); ++b)
+// Largest bucket of the single-wave kernel: variant 0 tiles of at most 64
+// lanes.  The multi-wave kernel takes the buckets above it.
+template <int DT> constexpr int seg_small_max() {
+    const int b = seg_loge<DT>(seg_max_logw<DT>(), 0) + 6;
+    return b < seg_max_logw<DT>() ? b : seg_max_logw<DT>() - 1;
+}
+
+// LDS words (keys; payload uses the same count of int32) a persistent
+// kernel for buckets BLO..BHI needs: the maximum over their variants.
+template <int DT, int BLO, int BHI> constexpr int seg_lds_words() {
+    int m = 1 << BHI;    // LDS-resident variant: one segment of W
+    for (int b = BLO; b <= BHI; ++b)
         for (int v = 0; v < 2; ++v)
             if (seg_variant_ok<DT>(b, v)) {
                 const int w = tile_lds_words<Elem<DT>>(b, seg_loge<DT>(b, v));
@@ -83,7 +95,7 @@ template <int DT> constexpr int seg_lds_words() {
 
 // LDS-resident network for one segment with all threads of the block.
 template <class E>
-__device__ void lds_segment_sort(const typename E::Storage* src, typename E::Storage* dst, int n, int logw,
+__device__ __forceinline__ void lds_segment_sort(const typename E::Storage* src, typename E::Storage* dst, int n, int logw,
                                  typename E::Key* lk, int32_t* lp) {
     const int W = 1 << logw;
     const int T = int(blockDim.x);
@@ -119,13 +131,29 @@ __device__ void lds_segment_sort(const typename E::Storage* src, typename E::Sto
     __syncthreads();
 }
 
+// The body of every (bucket, variant) is a separate, non-inlined function so
+// that each gets its own register allocation: inlined into one kernel, the
+// allocator sized the whole persistent kernel for its worst bucket (more
+// than 400 registers per lane, one wave per SIMD).  Their global pointers
+// carry the global address space explicitly, so that the loads and stores
+// inside stay global (not flat) instructions.
+template <class T> using global_ptr = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ global_ptr<T> to_global(T* p) { return (global_ptr<T>)p; }
+
 // One item: `count` segments of bucket B (list entries first..first+count-1)
 // with register-tile variant V.
 template <int DT, int B, int VAR>
-__device__ __forceinline__ void seg_tile_item(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* out,
-                                              const int64_t* off, const int* list, int first, int count,
-                                              typename Elem<DT>::Key* lk, int32_t* lp) {
+__device__ __attribute__((noinline)) void seg_tile_item(global_ptr<const typename Elem<DT>::Storage> gin,
+                                                        global_ptr<typename Elem<DT>::Storage> gout,
+                                                        global_ptr<const int64_t> goff, global_ptr<const int> glist,
+                                                        int first, int count, typename Elem<DT>::Key* lk,
+                                                        int32_t* lp) {
     using E = Elem<DT>;
+    using S = typename E::Storage;
+    const S* in = (const S*)gin;
+    S* out = (S*)gout;
+    const int64_t* off = (const int64_t*)goff;
+    const int* list = (const int*)glist;
     constexpr int LE = seg_loge<DT>(B, VAR);
     using C = TileCfg<E, B, LE>;
     const int tid = int(threadIdx.x);
@@ -139,54 +167,85 @@ __device__ __forceinline__ void seg_tile_item(const typename Elem<DT>::Storage* 
                         lp + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
 }
 
+// One item of the LDS-resident variant: the single segment list[first].
+template <int DT, int B>
+__device__ __attribute__((noinline)) void seg_lds_item(global_ptr<const typename Elem<DT>::Storage> gin,
+                                                       global_ptr<typename Elem<DT>::Storage> gout,
+                                                       global_ptr<const int64_t> goff, global_ptr<const int> glist,
+                                                       int first, typename Elem<DT>::Key* lk, int32_t* lp) {
+    using S = typename Elem<DT>::Storage;
+    const int s = ((const int*)glist)[first];
+    const int64_t* off = (const int64_t*)goff;
+    const int64_t b0 = off[s];
+    lds_segment_sort<Elem<DT>>((const S*)gin + b0, (S*)gout + b0, int(off[s + 1] - b0), B, lk, lp);
+}
+
 template <int DT, int B>
 __device__ __forceinline__ void seg_bucket(int variant, const typename Elem<DT>::Storage* in,
                                            typename Elem<DT>::Storage* out, const int64_t* off, const int* list,
                                            int first, int count, typename Elem<DT>::Key* lk, int32_t* lp) {
     if constexpr (seg_variant_ok<DT>(B, 0)) {
-        if (variant == 0) { seg_tile_item<DT, B, 0>(in, out, off, list, first, count, lk, lp); return; }
+        if (variant == 0) {
+            seg_tile_item<DT, B, 0>(to_global(in), to_global(out), to_global(off), to_global(list), first, count,
+                                    lk, lp);
+            return;
+        }
     }
     if constexpr (seg_variant_ok<DT>(B, 1)) {
-        if (variant == 1) { seg_tile_item<DT, B, 1>(in, out, off, list, first, count, lk, lp); return; }
+        if (variant == 1) {
+            seg_tile_item<DT, B, 1>(to_global(in), to_global(out), to_global(off), to_global(list), first, count,
+                                    lk, lp);
+            return;
+        }
     }
     if constexpr (seg_variant_ok<DT>(B, 2)) {
-        if (variant == 2) {
-            const int s = list[first];
-            const int64_t b0 = off[s];
-            lds_segment_sort<Elem<DT>>(in + b0, out + b0, int(off[s + 1] - b0), B, lk, lp);
-        }
+        if (variant == 2)
+            seg_lds_item<DT, B>(to_global(in), to_global(out), to_global(off), to_global(list), first, lk, lp);
     }
 }
 
-template <int DT, int B = 0>
+template <int DT, int B, int BHI>
 __device__ __forceinline__ void seg_dispatch(int bucket, int variant, const typename Elem<DT>::Storage* in,
                                              typename Elem<DT>::Storage* out, const int64_t* off, const int* list,
                                              int first, int count, typename Elem<DT>::Key* lk, int32_t* lp) {
-    if constexpr (B <= seg_max_logw<DT>()) {
+    if constexpr (B <= BHI) {
         if (bucket == B) {
             seg_bucket<DT, B>(variant, in, out, off, list, first, count, lk, lp);
             return;
         }
-        seg_dispatch<DT, B + 1>(bucket, variant, in, out, off, list, first, count, lk, lp);
+        seg_dispatch<DT, B + 1, BHI>(bucket, variant, in, out, off, list, first, count, lk, lp);
     }
 }
 
-template <int DT>
-__global__ void __launch_bounds__(kSegBlock)
+// Workgroups of a persistent kernel for buckets BLO..BHI that fit a CU's
+// 160 KB of LDS; each has one wave per SIMD, so this is also the number of
+// waves per SIMD the register allocation must leave room for (left alone,
+// the allocator sizes the kernel for one wave per SIMD).
+template <int DT, int BLO, int BHI> constexpr int seg_waves_per_simd() {
+    const int bytes = seg_lds_words<DT, BLO, BHI>() * int(sizeof(typename Elem<DT>::Key) +
+                                                          (Elem<DT>::kPayload ? 4 : 0));
+    const int w = (160 * 1024) / (bytes + 64);
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// Persistent kernel for the items [begin, end) of the list, whose buckets
+// all lie in BLO..BHI; *counter starts at 0.
+template <int DT, int BLO, int BHI>
+__global__ void __launch_bounds__(kSegBlock, (seg_waves_per_simd<DT, BLO, BHI>()))
 k_seg_persistent(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* out, const int64_t* off,
-                 const int* list, const SegItem* items, int nitems, int* counter) {
+                 const int* list, const SegItem* items, int begin, int end, int* counter) {
     using E = Elem<DT>;
-    __shared__ typename E::Key lds_k[seg_lds_words<DT>()];
-    __shared__ int32_t lds_p[E::kPayload ? seg_lds_words<DT>() : 1];
+    __shared__ typename E::Key lds_k[seg_lds_words<DT, BLO, BHI>()];
+    __shared__ int32_t lds_p[E::kPayload ? seg_lds_words<DT, BLO, BHI>() : 1];
     __shared__ int item_slot;
     for (;;) {
-        if (threadIdx.x == 0) item_slot = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) item_slot = begin + atomicAdd(counter, 1);
         __syncthreads();
         const int i = item_slot;
         __syncthreads();                       // everyone has read the slot before it is reused
-        if (i >= nitems) break;                // uniform: every wave leaves together
+        if (i >= end) break;                   // uniform: every wave leaves together
         const SegItem it = items[i];
-        seg_dispatch<DT>(it.bucket, it.variant, in, out, off, list, it.first, it.count, lds_k, lds_p);
+        seg_dispatch<DT, BLO, BHI>(it.bucket, it.variant, in, out, off, list, it.first, it.count, lds_k, lds_p);
     }
 }
 
@@ -212,13 +271,21 @@ k_seg_plain(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* ou
                         lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
 }
 
-__global__ void k_seg_reset(int* counter);
+// Zeroes the two item counters of the persistent kernels.
+__global__ void k_seg_reset(int* counters);
+
+// The two persistent kernels: kSegMulti for buckets above small_max (tiles
+// of several waves), kSegSingle for buckets 0..small_max.
+constexpr int kSegMulti = 0;
+constexpr int kSegSingle = 1;
 
 // Host side (segmented_<dtype>.hip).
 struct SegKernels {
-    hipError_t (*persistent)(const void* in, void* out, const int64_t* off, const int* list, const SegItem* items,
-                             int nitems, int* counter, int grid, hipStream_t s) = nullptr;
-    const void* persistent_fn = nullptr;
+    hipError_t (*persistent)(int which, const void* in, void* out, const int64_t* off, const int* list,
+                             const SegItem* items, int begin, int end, int* counter, int grid,
+                             hipStream_t s) = nullptr;
+    const void* persistent_fn[2] = {nullptr, nullptr};
+    int small_max = 0;   // largest bucket of the kSegSingle kernel
     hipError_t (*plain)(const void* in, void* out, const int64_t* off, int64_t segments, int bucket,
                         hipStream_t s) = nullptr;
     int max_logw = 0;
@@ -227,11 +294,18 @@ struct SegKernels {
 };
 
 template <int DT>
-hipError_t seg_launch_persistent(const void* in, void* out, const int64_t* off, const int* list, const SegItem* items,
-                                 int nitems, int* counter, int grid, hipStream_t s) {
+hipError_t seg_launch_persistent(int which, const void* in, void* out, const int64_t* off, const int* list,
+                                 const SegItem* items, int begin, int end, int* counter, int grid, hipStream_t s) {
     using S = typename Elem<DT>::Storage;
-    hipLaunchKernelGGL(k_seg_persistent<DT>, dim3(unsigned(grid)), dim3(kSegBlock), 0, s,
-                       static_cast<const S*>(in), static_cast<S*>(out), off, list, items, nitems, counter);
+    constexpr int SM = seg_small_max<DT>();
+    if (end <= begin || grid <= 0) return hipSuccess;
+    if (which == kSegMulti)
+        hipLaunchKernelGGL((k_seg_persistent<DT, SM + 1, seg_max_logw<DT>()>), dim3(unsigned(grid)), dim3(kSegBlock),
+                           0, s, static_cast<const S*>(in), static_cast<S*>(out), off, list, items, begin, end,
+                           counter);
+    else
+        hipLaunchKernelGGL((k_seg_persistent<DT, 0, SM>), dim3(unsigned(grid)), dim3(kSegBlock), 0, s,
+                           static_cast<const S*>(in), static_cast<S*>(out), off, list, items, begin, end, counter);
     return hipGetLastError();
 }
 
@@ -260,8 +334,12 @@ template <int DT> bool seg_ok(int b, int v) { return seg_variant_ok<DT>(b, v); }
 template <int DT>
 SegKernels seg_kernels() {
     SegKernels k;
+    constexpr int SM = seg_small_max<DT>();
     k.persistent = &seg_launch_persistent<DT>;
-    k.persistent_fn = reinterpret_cast<const void*>(&k_seg_persistent<DT>);
+    k.persistent_fn[kSegMulti] =
+        reinterpret_cast<const void*>(&k_seg_persistent<DT, SM + 1, seg_max_logw<DT>()>);
+    k.persistent_fn[kSegSingle] = reinterpret_cast<const void*>(&k_seg_persistent<DT, 0, SM>);
+    k.small_max = SM;
     k.plain = &seg_launch_plain<DT>;
     k.max_logw = seg_max_logw<DT>();
     k.segments_per_item = &seg_spi<DT>;

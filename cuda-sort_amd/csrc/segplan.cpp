@@ -14,8 +14,8 @@
 
 namespace cs {
 
-__global__ void k_seg_reset(int* counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+__global__ void k_seg_reset(int* counters) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0;
 }
 
 SegKernels seg_kernels_i32();
@@ -81,31 +81,33 @@ struct cs_segplan {
     int64_t total = 0;
     int64_t max_len = 0;
     int nitems = 0;
-    int grid = 0;
+    SegShape shape;
     int variant[kMaxBuckets] = {};
     int64_t bucket_segments[kMaxBuckets] = {};
     SegKernels k;
     int* d_list = nullptr;
     SegItem* d_items = nullptr;
-    int* d_counter = nullptr;
+    int* d_counter = nullptr;          // two item counters
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraphNode_t reset_node = nullptr;
-    hipGraphNode_t sort_node = nullptr;
-    // arguments baked into the graph's sort node
+    hipGraphNode_t sort_node[2] = {nullptr, nullptr};
+    // arguments baked into the graph's nodes
     const void* g_in = nullptr;
     void* g_out = nullptr;
     const int64_t* g_off = nullptr;
-    void* args[7] = {};
+    void* args[2][8] = {};
     void* reset_args[1] = {};
     const void* a_in = nullptr;
     void* a_out = nullptr;
     const int64_t* a_off = nullptr;
     const int* a_list = nullptr;
     const SegItem* a_items = nullptr;
-    int a_nitems = 0;
-    int* a_counter = nullptr;
+    int a_begin[2] = {0, 0};
+    int a_end[2] = {0, 0};
+    int* a_counter[2] = {nullptr, nullptr};
+    int* a_reset = nullptr;
 };
 
 namespace {
@@ -120,14 +122,27 @@ void free_plan(cs_segplan* p) {
     delete p;
 }
 
+hipKernelNodeParams sort_node_params(cs_segplan* p, int w) {
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void*>(p->k.persistent_fn[w]);
+    kp.gridDim = dim3(unsigned(p->shape.grid[w]));
+    kp.blockDim = dim3(kSegBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = p->args[w];
+    kp.extra = nullptr;
+    return kp;
+}
+
+// The graph: counter reset (a kernel node, not a memset node), then the
+// two persistent kernels as independent nodes, free to run concurrently.
 hipError_t build_graph(cs_segplan* p, const void* in, void* out, const int64_t* off) {
     if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
     if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+    p->sort_node[0] = p->sort_node[1] = nullptr;
     hipError_t e = hipGraphCreate(&p->graph, 0);
     if (e != hipSuccess) return e;
-    // Counter reset is a kernel node, not a memset node.
-    p->a_counter = p->d_counter;
-    p->reset_args[0] = &p->a_counter;
+    p->a_reset = p->d_counter;
+    p->reset_args[0] = &p->a_reset;
     hipKernelNodeParams rp{};
     rp.func = reinterpret_cast<void*>(&k_seg_reset);
     rp.gridDim = dim3(1);
@@ -138,18 +153,19 @@ hipError_t build_graph(cs_segplan* p, const void* in, void* out, const int64_t* 
     e = hipGraphAddKernelNode(&p->reset_node, p->graph, nullptr, 0, &rp);
     if (e != hipSuccess) return e;
     p->a_in = in; p->a_out = out; p->a_off = off;
-    p->a_list = p->d_list; p->a_items = p->d_items; p->a_nitems = p->nitems;
-    p->args[0] = &p->a_in; p->args[1] = &p->a_out; p->args[2] = &p->a_off; p->args[3] = &p->a_list;
-    p->args[4] = &p->a_items; p->args[5] = &p->a_nitems; p->args[6] = &p->a_counter;
-    hipKernelNodeParams kp{};
-    kp.func = const_cast<void*>(p->k.persistent_fn);
-    kp.gridDim = dim3(unsigned(p->grid));
-    kp.blockDim = dim3(kSegBlock);
-    kp.sharedMemBytes = 0;
-    kp.kernelParams = p->args;
-    kp.extra = nullptr;
-    e = hipGraphAddKernelNode(&p->sort_node, p->graph, &p->reset_node, 1, &kp);
-    if (e != hipSuccess) return e;
+    p->a_list = p->d_list; p->a_items = p->d_items;
+    for (int w = 0; w < 2; ++w) {
+        p->a_begin[w] = w == kSegMulti ? 0 : p->shape.split;
+        p->a_end[w] = w == kSegMulti ? p->shape.split : p->nitems;
+        p->a_counter[w] = p->d_counter + w;
+        void** a = p->args[w];
+        a[0] = &p->a_in; a[1] = &p->a_out; a[2] = &p->a_off; a[3] = &p->a_list; a[4] = &p->a_items;
+        a[5] = &p->a_begin[w]; a[6] = &p->a_end[w]; a[7] = &p->a_counter[w];
+        if (p->a_end[w] <= p->a_begin[w]) continue;
+        const hipKernelNodeParams kp = sort_node_params(p, w);
+        e = hipGraphAddKernelNode(&p->sort_node[w], p->graph, &p->reset_node, 1, &kp);
+        if (e != hipSuccess) return e;
+    }
     e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) return e;
     p->g_in = in; p->g_out = out; p->g_off = off;
@@ -158,14 +174,12 @@ hipError_t build_graph(cs_segplan* p, const void* in, void* out, const int64_t* 
 
 hipError_t update_graph(cs_segplan* p, const void* in, void* out, const int64_t* off) {
     p->a_in = in; p->a_out = out; p->a_off = off;
-    hipKernelNodeParams kp{};
-    kp.func = const_cast<void*>(p->k.persistent_fn);
-    kp.gridDim = dim3(unsigned(p->grid));
-    kp.blockDim = dim3(kSegBlock);
-    kp.sharedMemBytes = 0;
-    kp.kernelParams = p->args;
-    kp.extra = nullptr;
-    hipError_t e = hipGraphExecKernelNodeSetParams(p->exec, p->sort_node, &kp);
+    hipError_t e = hipSuccess;
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) {
+        if (!p->sort_node[w]) continue;
+        const hipKernelNodeParams kp = sort_node_params(p, w);
+        e = hipGraphExecKernelNodeSetParams(p->exec, p->sort_node[w], &kp);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return build_graph(p, in, out, off);   // fall back to a fresh instantiation
@@ -192,11 +206,11 @@ int make_plan_impl(cs_segplan** out, int dtype, const int64_t* h_offsets, int64_
     if (rc0 != CS_OK) { delete p; return rc0; }
     p->total = segments ? h_offsets[segments] - h_offsets[0] : 0;
     p->nitems = int(items.size());
-    p->grid = seg_persistent_grid(dtype, p->nitems);
-    if (p->grid <= 0) { free_plan(p); return CS_E_NO_DEVICE; }
+    p->shape = seg_shape(dtype, items);
+    if (p->shape.grid[0] <= 0 || p->shape.grid[1] <= 0) { free_plan(p); return CS_E_NO_DEVICE; }
     hipError_t e = hipMalloc(&p->d_list, std::max<size_t>(list.size(), 1) * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&p->d_items, std::max<size_t>(items.size(), 1) * sizeof(SegItem));
-    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, 2 * sizeof(int));
     if (e == hipSuccess && !list.empty())
         e = hipMemcpy(p->d_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !items.empty())
@@ -217,10 +231,8 @@ int run_plan(cs_segplan* p, const void* in, void* out, const int64_t* d_offsets,
         else e = hipSuccess;
         if (e == hipSuccess) e = hipGraphLaunch(p->exec, stream);
     } else {
-        hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, stream, p->d_counter);
-        e = hipGetLastError();
-        if (e == hipSuccess)
-            e = p->k.persistent(in, out, d_offsets, p->d_list, p->d_items, p->nitems, p->d_counter, p->grid, stream);
+        e = seg_launch_all(p->dtype, p->shape, in, out, d_offsets, p->d_list, p->d_items, p->nitems, p->d_counter,
+                           stream);
     }
     return e == hipSuccess ? CS_OK : int(e);
 }
@@ -268,21 +280,46 @@ int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* fo
     return CS_OK;
 }
 
-int seg_persistent_grid(int dtype, int nitems) {
+int seg_persistent_grid(int dtype, int which, int nitems) {
     static std::mutex mu;
-    static int cache[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    static int cache[8][2] = {};
     std::lock_guard<std::mutex> lock(mu);
-    if (!cache[dtype]) {
-        int dev = 0, cus = 1, per_cu = 1;
+    if (dtype < 0 || dtype >= 8 || which < 0 || which > 1) return 0;
+    if (!cache[dtype][which]) {
+        int dev = 0, cus = 0, per_cu = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, seg_kernels_for(dtype).persistent_fn,
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, seg_kernels_for(dtype).persistent_fn[which],
                                                               kSegBlock, 0);
         if (e != hipSuccess) { (void)hipGetLastError(); return 0; }
-        cache[dtype] = std::max(1, cus * std::max(per_cu, 1));
+        cache[dtype][which] = std::max(1, cus * std::max(per_cu, 1));
     }
-    return std::max(1, std::min(cache[dtype], std::max(nitems, 1)));
+    return std::max(1, std::min(cache[dtype][which], std::max(nitems, 1)));
+}
+
+SegShape seg_shape(int dtype, const std::vector<SegItem>& items) {
+    SegShape sh;
+    const int sm = seg_kernels_for(dtype).small_max;
+    sh.split = 0;
+    while (sh.split < int(items.size()) && items[size_t(sh.split)].bucket > sm) ++sh.split;
+    sh.grid[kSegMulti] = seg_persistent_grid(dtype, kSegMulti, sh.split);
+    sh.grid[kSegSingle] = seg_persistent_grid(dtype, kSegSingle, int(items.size()) - sh.split);
+    return sh;
+}
+
+hipError_t seg_launch_all(int dtype, const SegShape& sh, const void* in, void* out, const int64_t* off,
+                          const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counters);
+    hipError_t e = hipGetLastError();
+    const SegKernels k = seg_kernels_for(dtype);
+    if (e == hipSuccess)
+        e = k.persistent(kSegMulti, in, out, off, d_list, d_items, 0, sh.split, d_counters, sh.grid[kSegMulti], s);
+    if (e == hipSuccess)
+        e = k.persistent(kSegSingle, in, out, off, d_list, d_items, sh.split, nitems, d_counters + 1,
+                         sh.grid[kSegSingle], s);
+    return e;
 }
 
 int ragged_prepare(int dtype, const int64_t* h_off, int64_t segs, RaggedPrep* p) {
@@ -306,18 +343,15 @@ int ragged_prepare(int dtype, const int64_t* h_off, int64_t segs, RaggedPrep* p)
     const int rc = seg_build_items(dtype, p->chunk_off.data(), int64_t(p->chunk_off.size()) - 1, nullptr, p->list,
                                    p->items, nullptr, nullptr, nullptr);
     if (rc != CS_OK) return rc;
-    p->grid = seg_persistent_grid(dtype, int(p->items.size()));
-    return p->grid > 0 ? CS_OK : CS_E_NO_DEVICE;
+    p->shape = seg_shape(dtype, p->items);
+    return p->shape.grid[0] > 0 && p->shape.grid[1] > 0 ? CS_OK : CS_E_NO_DEVICE;
 }
 
 hipError_t ragged_launch(int dtype, const void* in, void* out, void* scratch, const int64_t* d_chunk_off,
-                         const int* d_list, const SegItem* d_items, int nitems, int* d_counter, int grid,
-                         MergeTile* d_tiles, int ntiles, int passes, hipStream_t s) {
+                         const int* d_list, const SegItem* d_items, int nitems, int* d_counters,
+                         const SegShape& shape, MergeTile* d_tiles, int ntiles, int passes, hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counter);
-    hipError_t e = hipGetLastError();
-    const SegKernels k = seg_kernels_for(dtype);
-    if (e == hipSuccess) e = k.persistent(in, out, d_chunk_off, d_list, d_items, nitems, d_counter, grid, s);
+    hipError_t e = seg_launch_all(dtype, shape, in, out, d_chunk_off, d_list, d_items, nitems, d_counters, s);
     if (e != hipSuccess || ntiles <= 0 || passes <= 0) return e;
     const PartKernels pk = part_kernels_for(dtype);
     const int64_t chunk = large_kernels_for(dtype).chunk;
@@ -353,7 +387,8 @@ int cs_segmented_plan_info(const cs_segplan* plan, int32_t* work_items, int32_t*
                            int32_t* variants, int64_t* bucket_segments, int32_t buckets) {
     if (!plan) return CS_E_INVALID_VALUE;
     if (work_items) *work_items = plan->nitems;
-    if (grid) *grid = plan->grid;
+    if (grid) *grid = plan->shape.grid[kSegMulti] * (plan->shape.split > 0) +
+                      plan->shape.grid[kSegSingle] * (plan->nitems > plan->shape.split);
     if (max_len) *max_len = plan->max_len;
     for (int b = 0; b < buckets && b < kMaxBuckets; ++b) {
         if (variants) variants[b] = plan->variant[b];

@@ -64,7 +64,7 @@ struct Slot {
     int* d_list = nullptr;
     SegItem* d_items = nullptr;
     MergeTile* d_tiles = nullptr;
-    int* d_counter = nullptr;
+    int* d_counter = nullptr;   // two item counters of the persistent kernels
     size_t pool_off = 0;
     std::vector<std::shared_ptr<Request>> reqs;
     std::vector<int64_t> req_off;
@@ -172,7 +172,7 @@ void launch_batch(cs_server* s, Slot& sl) {
                            sl.stream);
     if (e == hipSuccess)
         e = ragged_launch(s->dtype, data, data, scratch, sl.d_off, sl.d_list, sl.d_items, int(prep.items.size()),
-                          sl.d_counter, prep.grid, sl.d_tiles, int(prep.tiles.size()), prep.passes, sl.stream);
+                          sl.d_counter, prep.shape, sl.d_tiles, int(prep.tiles.size()), prep.passes, sl.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(sl.h_out, data, bytes, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
     sl.status = e == hipSuccess ? CS_OK : int(e);
@@ -331,9 +331,10 @@ int cs_server_create(cs_server** out, int dtype, size_t budget_bytes, int32_t sl
         if (e == hipSuccess) e = hipMalloc(&sl.d_list, size_t(s->max_chunks + 1) * 4);
         if (e == hipSuccess) e = hipMalloc(&sl.d_items, size_t(s->max_chunks + 1) * sizeof(SegItem));
         if (e == hipSuccess) e = hipMalloc(&sl.d_tiles, size_t(s->max_tiles + 1) * sizeof(MergeTile));
-        if (e == hipSuccess) e = hipMalloc(&sl.d_counter, sizeof(int));
+        if (e == hipSuccess) e = hipMalloc(&sl.d_counter, 2 * sizeof(int));
     }
-    if (e == hipSuccess && seg_persistent_grid(dtype, 1) <= 0) e = hipErrorNoDevice;
+    if (e == hipSuccess && (seg_persistent_grid(dtype, kSegMulti, 1) <= 0 || seg_persistent_grid(dtype, kSegSingle, 1) <= 0))
+        e = hipErrorNoDevice;
     if (e != hipSuccess) {
         destroy_server(s);
         return int(e);
