@@ -237,7 +237,8 @@ def bench_ragged(args, dtype_name: str):
             "plain_per_array": {"ms_per_step": ms_b, "value": total / (ms_b * 1e-3) / 1e9,
                                 "speedup_persistent_vs_plain": ms_b / ms_p, "identical_output": same},
             "roofline": {"bound": "hbm", "achieved": alg / (ms_p * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": alg / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+                         "frac": alg / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": load_traffic(f"ragged_{dtype_name}_{args.segments}x{args.max_len}")},
             "cpu_baseline": cpu, "validated": ok}
 
 
@@ -292,14 +293,25 @@ def bench_large(args, dtype_name: str):
         secs, threads = oracle.time_large(dt, host_in, threads=0, trials=1)
         cpu = {"value": n / secs / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
                "sample": f"the full array (2^{log2n} keys), __gnu_parallel::sort over {threads} host threads, 1 trial"}
-    alg = 2 * nbytes
+    # Algorithmic HBM bytes: the merge path streams the whole array once for
+    # the chunk sort and once per merge pass (read + write every key); the
+    # partition path is given the whole-sort lower bound (one read + one
+    # write), since its pass count depends on the bucket sizes.
+    merge_like = args.large_method == "merge" or (args.large_method == "hybrid" and n < cs.hybrid_threshold(dt))
+    chunk = 8192 if cs.DTYPE_SIZE[dt] == 4 else 4096
+    passes = max(0, (n - 1).bit_length() - (chunk - 1).bit_length()) if merge_like else 0
+    alg = (passes + 1) * 2 * nbytes if merge_like else 2 * nbytes
+    alg_note = (f"chunk sort + {passes} merge passes, each reading and writing all {n} keys" if merge_like
+                else "one read and one write of every key (whole-sort lower bound)")
     return {"metric": METRIC, "value": n / (ms * 1e-3) / 1e9, "unit": "Gkeys/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "int32" if dt == cs.I32 else "uint64", "data": "synthetic",
             "config": {"workload": f"large-array sort, 2^{log2n} {dtype_name} keys, {args.large_dist}, seed 42, "
                                    f"{args.large_method} path", "n": n, "method": args.large_method},
             "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+                         "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": load_traffic(f"large_{dtype_name}_2p{log2n}_{'merge' if merge_like else 'partition'}"),
+                         "algorithmic_bytes": alg, "basis": alg_note},
             "cpu_baseline": cpu, "validated": ok}
 
 
