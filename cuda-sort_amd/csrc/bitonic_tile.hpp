@@ -237,11 +237,14 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
 
     // ---- load, lane-striped.  When every group of this lane is in range
     // (all lanes of a full array), the loads form one branch-free block, all
-    // in flight together; otherwise each group is bounds checked.  Stores
-    // are split the same way.
+    // in flight together: 16-byte vectors when the array is 16-byte aligned,
+    // element loads otherwise (ragged segments start anywhere).  A lane
+    // with a group past the end bounds-checks each group.  Stores are split
+    // the same way.
     constexpr int NG = NE / VS;
-    const bool full_in = vec_in && ((NG - 1) * T + t) * VS + VS <= valid;
-    const bool full_out = vec_out && ((NG - 1) * T + t) * VS + VS <= valid;
+    const bool full = ((NG - 1) * T + t) * VS + VS <= valid;   // every group of this lane in range
+    const bool full_in = vec_in && full;
+    const bool full_out = vec_out && full;
     if (full_in) {
         sfor<0, NG>([&](auto q_) {
             constexpr int q = decltype(q_)::value;
@@ -252,6 +255,16 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
                 constexpr int j = decltype(j_)::value;
                 tile.k[q * VS + j] = E::key(tmp[j]);
                 tile.p[q * VS + j] = E::payload(tmp[j]);
+            });
+        });
+    } else if (full) {   // in range but not 16-byte aligned: element loads, no branches
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            sfor<0, VS>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                const Storage s = src[(q * T + t) * VS + j];
+                tile.k[q * VS + j] = E::key(s);
+                tile.p[q * VS + j] = E::payload(s);
             });
         });
     } else {
@@ -302,6 +315,14 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
     };
     if (full_out) {
         sfor<0, NG>(store_group);
+    } else if (full) {
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            sfor<0, VS>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                dst[(q * T + t) * VS + j] = E::make(tile.k[q * VS + j], tile.p[q * VS + j]);
+            });
+        });
     } else {
         sfor<0, NG>([&](auto q_) {
             constexpr int q = decltype(q_)::value;

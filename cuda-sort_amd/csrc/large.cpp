@@ -31,8 +31,8 @@ LargeKernels large_kernels_for(int dtype) {
     }
 }
 
-int partition_sort(int dtype, void* data, int64_t n, void* out, void* scratch, size_t scratch_elems, int max_parts,
-                   hipStream_t stream);
+int partition_sort(int dtype, void* data, int64_t n, void* out, void* scratch, size_t scratch_elems, int min_parts,
+                   int max_parts, hipStream_t stream);
 
 }  // namespace cs
 
@@ -170,7 +170,7 @@ int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32
     void* tmp = nullptr;
     hipError_t e = hipMallocAsync(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
     if (e != hipSuccess) return int(e);
-    const int rc = partition_sort(dtype, data, n, data, tmp, size_t(n), max_parts, stream);
+    const int rc = partition_sort(dtype, data, n, data, tmp, size_t(n), min_parts, max_parts, stream);
     e = hipFreeAsync(tmp, stream);
     return rc != CS_OK ? rc : (e == hipSuccess ? CS_OK : int(e));
 }
@@ -192,7 +192,7 @@ int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream
     }
     int rc;
     if (n < threshold) rc = merge_sort(dtype, data, n, data, scratch, stream);
-    else rc = partition_sort(dtype, data, n, data, scratch, size_t(n), 0, stream);
+    else rc = partition_sort(dtype, data, n, data, scratch, size_t(n), 0, 0, stream);
     if (owned) {
         const hipError_t e = hipFreeAsync(scratch, stream);
         if (rc == CS_OK && e != hipSuccess) rc = int(e);

@@ -159,16 +159,24 @@ __device__ __forceinline__ void merge_tile_body(const typename Elem<DT>::Storage
     S* out = dst + g.base + g.d0;
     const bool vec = (((uintptr_t)out) & 15) == 0 && V > 1;
     for (int q = tid * V; q < total; q += kMergeThreads * V) {
-        if (vec && q + V <= total) {
-            S tmp[V];
-            #pragma unroll
-            for (int v = 0; v < V; ++v) tmp[v] = lds[(q + v) + ((q + v) >> PAD)];
-            using Vec = __attribute__((ext_vector_type(4))) uint32_t;
-            Vec w;
-            __builtin_memcpy(&w, tmp, 16);
-            *reinterpret_cast<Vec*>(out + q) = w;
+        S tmp[V];
+        #pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int x = q + v < total ? q + v : q;
+            tmp[v] = lds[x + (x >> PAD)];
+        }
+        if (q + V <= total) {
+            if (vec) {   // 16-byte aligned tile start (plain merge passes)
+                using Vec = __attribute__((ext_vector_type(4))) uint32_t;
+                Vec w;
+                __builtin_memcpy(&w, tmp, 16);
+                *reinterpret_cast<Vec*>(out + q) = w;
+            } else {     // tiles of buckets and requests start anywhere
+                #pragma unroll
+                for (int v = 0; v < V; ++v) out[q + v] = tmp[v];
+            }
         } else {
-            for (int v = 0; v < V && q + v < total; ++v) out[q + v] = lds[(q + v) + ((q + v) >> PAD)];
+            for (int v = 0; v < V && q + v < total; ++v) out[q + v] = tmp[v];
         }
     }
 }

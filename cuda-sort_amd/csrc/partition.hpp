@@ -13,14 +13,17 @@
 //     histogram and writes column (block) of a bucket-major count matrix.
 //  4. scan: exclusive prefix sum of the matrix (bucket-major, block-minor)
 //     gives each (bucket, block) its output offset.
-//  5. scatter: the workgroup re-classifies its block and places every
-//     element at its bucket's next slot (LDS atomics; order inside a bucket
-//     is free, the sort is unstable like the reference's).
+//  5. scatter: the workgroup re-classifies its block tile by tile, stages
+//     each tile in bucket order in LDS (ranks from LDS atomics; order inside
+//     a bucket is free, the sort is unstable like the reference's) and
+//     writes each bucket's run of the tile to consecutive addresses.
 // The host then reads the bucket sizes once (the reference also reads its
 // counters back, cusort.hpp:1967-1977) and sorts the buckets as one ragged
 // batch; buckets longer than the on-chip tile are cut into chunks whose
 // sorted runs are merged by tiled merge passes over a tile list.
 #pragma once
+
+#include <type_traits>
 
 #include "large.hpp"
 
@@ -53,9 +56,49 @@ __global__ void k_part_splitters(const typename Elem<DT>::Storage* sorted_sample
     if (j < parts - 1) splitters[j] = Elem<DT>::key(sorted_samples[int64_t(j + 1) * m / parts - 1]);
 }
 
+// Elements per thread per tile of the count and scatter kernels.
+template <int DT> constexpr int part_items() { return sizeof(typename Elem<DT>::Storage) == 4 ? 16 : 8; }
+template <int DT> constexpr int part_tile() { return part_items<DT>() * kPartThreads; }
+
+// Classification.  A lookup table indexed by the top kPartLutBits bits of
+// the key's order-preserving unsigned image narrows the splitter search:
+// lut[x] = number of splitters whose prefix is below x, so the splitters
+// that can tie with or bracket a key of prefix x are sp[lut[x] .. lut[x+1]).
+// For spread-out keys that range holds zero to a few splitters; for keys
+// crowded under one prefix it is the full list and the search is the plain
+// binary search.  Built by every workgroup from the splitters in LDS.
+constexpr int kPartLutBits = 12;
+constexpr int kPartLutSize = (1 << kPartLutBits) + 1;
+
+template <class K> __device__ __forceinline__ uint32_t part_prefix(K k) {
+    if constexpr (sizeof(K) == 4) {
+        const uint32_t u = std::is_signed<K>::value ? uint32_t(k) ^ 0x80000000u : uint32_t(k);
+        return u >> (32 - kPartLutBits);
+    } else {
+        const uint64_t u = std::is_signed<K>::value ? uint64_t(k) ^ (uint64_t(1) << 63) : uint64_t(k);
+        return uint32_t(u >> (64 - kPartLutBits));
+    }
+}
+
+template <class K>
+__device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint16_t* lut) {
+    for (int x = int(threadIdx.x); x < kPartLutSize; x += int(blockDim.x)) {
+        int lo = 0, hi = ns;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (part_prefix(sp[mid]) < uint32_t(x)) lo = mid + 1;
+            else hi = mid;
+        }
+        lut[x] = uint16_t(lo);
+    }
+}
+
+// Bucket of key k: j = number of splitters <= k; a key equal to splitter
+// j-1 goes to the equality bucket 2j-1, any other key to bucket 2j.
 template <class E>
-__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, int ns) {
-    int lo = 0, hi = ns;                 // j = number of splitters <= k
+__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, const uint16_t* lut) {
+    const uint32_t x = part_prefix(k);
+    int lo = lut[x], hi = lut[x + 1];
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (!(k < sp[mid])) lo = mid + 1;
@@ -64,44 +107,145 @@ __device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::
     return (lo > 0 && !(sp[lo - 1] < k)) ? 2 * lo - 1 : 2 * lo;
 }
 
+// Per-block histogram: every thread classifies part_items keys per tile,
+// their loads all in flight together (coalesced across the workgroup).
 template <int DT>
 __global__ void __launch_bounds__(kPartThreads)
 k_part_count(const typename Elem<DT>::Storage* src, int64_t n, const typename Elem<DT>::Key* splitters, int ns,
              int64_t per_block, int blocks, int64_t* counts) {
     using E = Elem<DT>;
+    constexpr int IT = part_items<DT>();
     __shared__ typename E::Key sp[kPartMaxSplitters];
     __shared__ int hist[kPartMaxBuckets];
+    __shared__ uint16_t lut[kPartLutSize];
     const int nb = 2 * ns + 1;
-    for (int i = int(threadIdx.x); i < ns; i += kPartThreads) sp[i] = splitters[i];
-    for (int i = int(threadIdx.x); i < nb; i += kPartThreads) hist[i] = 0;
+    const int tid = int(threadIdx.x);
+    for (int i = tid; i < ns; i += kPartThreads) sp[i] = splitters[i];
+    for (int i = tid; i < nb; i += kPartThreads) hist[i] = 0;
+    __syncthreads();
+    part_build_lut(sp, ns, lut);
     __syncthreads();
     const int64_t b0 = int64_t(blockIdx.x) * per_block;
     const int64_t b1 = b0 + per_block < n ? b0 + per_block : n;
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += kPartThreads)
-        atomicAdd(&hist[part_bucket<E>(E::key(src[i]), sp, ns)], 1);
+    for (int64_t t0 = b0; t0 < b1; t0 += int64_t(IT) * kPartThreads) {
+        typename E::Key key[IT];
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int64_t i = t0 + int64_t(k) * kPartThreads + tid;
+            key[k] = i < b1 ? E::key(src[i]) : typename E::Key{};
+        }
+        #pragma unroll
+        for (int k = 0; k < IT; ++k)
+            if (t0 + int64_t(k) * kPartThreads + tid < b1) atomicAdd(&hist[part_bucket<E>(key[k], sp, lut)], 1);
+    }
     __syncthreads();
-    for (int b = int(threadIdx.x); b < nb; b += kPartThreads) counts[int64_t(b) * blocks + blockIdx.x] = hist[b];
+    for (int b = tid; b < nb; b += kPartThreads) counts[int64_t(b) * blocks + blockIdx.x] = hist[b];
 }
 
+// Dynamic LDS of k_part_scatter for nb buckets: splitters, tile histogram /
+// offsets, the block's running output positions, the tile staged in bucket
+// order and the bucket of every staged element.
+constexpr size_t part_align(size_t x) { return (x + 15) & ~size_t(15); }
+template <int DT> size_t part_scatter_lds(int ns) {
+    using E = Elem<DT>;
+    const size_t nb = size_t(2 * ns + 1);
+    return part_align(size_t(ns > 0 ? ns : 1) * sizeof(typename E::Key)) + part_align(nb * 4) +
+           part_align(nb * 8) + part_align(size_t(part_tile<DT>()) * sizeof(typename E::Storage)) +
+           part_align(size_t(part_tile<DT>()) * 2) + 64;
+}
+
+// Scatter.  Per tile of part_tile elements: classify, rank inside the tile
+// with LDS atomics, scan the tile histogram, stage the tile in bucket order
+// in LDS, then write it out with consecutive threads on consecutive
+// positions of each bucket's run (coalesced where the reference's
+// per-element atomics scatter every key to its own address).
 template <int DT>
 __global__ void __launch_bounds__(kPartThreads)
 k_part_scatter(const typename Elem<DT>::Storage* src, int64_t n, const typename Elem<DT>::Key* splitters, int ns,
                int64_t per_block, int blocks, const int64_t* offsets, typename Elem<DT>::Storage* dst) {
     using E = Elem<DT>;
-    __shared__ typename E::Key sp[kPartMaxSplitters];
-    __shared__ unsigned long long next[kPartMaxBuckets];
+    using K = typename E::Key;
+    using S = typename E::Storage;
+    constexpr int IT = part_items<DT>();
+    constexpr int TILE = part_tile<DT>();
+    extern __shared__ __align__(16) unsigned char part_lds[];
     const int nb = 2 * ns + 1;
-    for (int i = int(threadIdx.x); i < ns; i += kPartThreads) sp[i] = splitters[i];
-    for (int b = int(threadIdx.x); b < nb; b += kPartThreads)
-        next[b] = (unsigned long long)offsets[int64_t(b) * blocks + blockIdx.x];
+    K* sp = reinterpret_cast<K*>(part_lds);
+    int* toff = reinterpret_cast<int*>(part_lds + part_align(size_t(ns > 0 ? ns : 1) * sizeof(K)));
+    int64_t* run = reinterpret_cast<int64_t*>(reinterpret_cast<unsigned char*>(toff) + part_align(size_t(nb) * 4));
+    S* stage = reinterpret_cast<S*>(reinterpret_cast<unsigned char*>(run) + part_align(size_t(nb) * 8));
+    uint16_t* sbkt = reinterpret_cast<uint16_t*>(reinterpret_cast<unsigned char*>(stage) +
+                                                 part_align(size_t(TILE) * sizeof(S)));
+    __shared__ int wsum[kPartThreads / 64];
+    __shared__ uint16_t lut[kPartLutSize];
+    const int tid = int(threadIdx.x);
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < ns; i += kPartThreads) sp[i] = splitters[i];
+    for (int b = tid; b < nb; b += kPartThreads) {
+        toff[b] = 0;
+        run[b] = offsets[int64_t(b) * blocks + blockIdx.x];
+    }
     __syncthreads();
+    part_build_lut(sp, ns, lut);
+    __syncthreads();
+    const int per_thread = (nb + kPartThreads - 1) / kPartThreads;   // scan chunk of each thread
     const int64_t b0 = int64_t(blockIdx.x) * per_block;
     const int64_t b1 = b0 + per_block < n ? b0 + per_block : n;
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += kPartThreads) {
-        const typename E::Storage s = src[i];
-        const int b = part_bucket<E>(E::key(s), sp, ns);
-        const unsigned long long pos = atomicAdd(&next[b], 1ull);
-        dst[pos] = s;
+    for (int64_t t0 = b0; t0 < b1; t0 += TILE) {
+        const int tn = int(b1 - t0 < TILE ? b1 - t0 : TILE);
+        S elem[IT];
+        int bkt[IT], rank[IT];
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int x = k * kPartThreads + tid;
+            elem[k] = x < tn ? src[t0 + x] : S{};
+        }
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int x = k * kPartThreads + tid;
+            bkt[k] = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut) : 0;
+            rank[k] = x < tn ? atomicAdd(&toff[bkt[k]], 1) : 0;
+        }
+        __syncthreads();
+        // exclusive scan of the tile histogram in place: a chunk per thread,
+        // then a wave scan of the chunk sums and the wave totals
+        const int c0 = tid * per_thread;
+        int csum = 0;
+        for (int b = c0; b < c0 + per_thread && b < nb; ++b) csum += toff[b];
+        int incl = csum;
+        #pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int acc = incl - csum;
+        for (int w = 0; w < wave; ++w) acc += wsum[w];
+        for (int b = c0; b < c0 + per_thread && b < nb; ++b) {
+            const int c = toff[b];
+            toff[b] = acc;
+            acc += c;
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            if (k * kPartThreads + tid < tn) {
+                const int pos = toff[bkt[k]] + rank[k];
+                stage[pos] = elem[k];
+                sbkt[pos] = uint16_t(bkt[k]);
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < tn; j += kPartThreads) {
+            const int b = sbkt[j];
+            dst[run[b] + (j - toff[b])] = stage[j];
+        }
+        __syncthreads();
+        for (int b = tid; b < nb; b += kPartThreads) run[b] += (b + 1 < nb ? toff[b + 1] : tn) - toff[b];
+        __syncthreads();   // every toff[b + 1] read before any is cleared
+        for (int b = tid; b < nb; b += kPartThreads) toff[b] = 0;
+        __syncthreads();
     }
 }
 
@@ -200,7 +344,13 @@ PartKernels part_kernels() {
     };
     k.scatter = [](const void* src, int64_t n, const void* sp, int ns, int64_t per_block, int blocks,
                    const int64_t* offsets, void* dst, hipStream_t s) {
-        hipLaunchKernelGGL(k_part_scatter<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), 0, s,
+        const size_t lds = part_scatter_lds<DT>(ns);
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_scatter<DT>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_part_scatter<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), lds, s,
                            static_cast<const S*>(src), n, static_cast<const K*>(sp), ns, per_block, blocks, offsets,
                            static_cast<S*>(dst));
         return hipGetLastError();
