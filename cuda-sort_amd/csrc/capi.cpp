@@ -110,7 +110,11 @@ int choose(int dtype, int32_t n, int method, int32_t threads, uint32_t flags, Ch
         case CS_BITONIC_V4_XOR: {
             if (n < 32) return CS_E_UNSUPPORTED;
             const int t = threads ? threads : (tmax < 256 ? tmax : 256);
-            if (t % 32 != 0 || t < 32 || t > tmax) return CS_E_UNSUPPORTED;
+            // v3 takes any multiple of 32 up to the block limit, also above W
+            // (the reference's historical tests run it with 32..512 threads
+            // at every size, test.cu:319-355); v4 keeps bench.cu's <= W.
+            const int tcap = method == CS_BITONIC_V3 ? 1024 : tmax;
+            if (t % 32 != 0 || t < 32 || t > tcap) return CS_E_UNSUPPORTED;
             return use_lds(logref, t);
         }
         case CS_BITONIC_V5_STATIC: {

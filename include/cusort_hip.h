@@ -107,7 +107,10 @@ int cs_method_from_name(const char* name);
  * `threads` follows the reference's per-method domain (0 = method default):
  *   bitonic_warp        n <= 32, threads 32
  *   bitonic_v1, v2      threads == W, W <= 1024
- *   bitonic_v3, v4_xor  threads a multiple of 32 in [32, min(W, 1024)]
+ *   bitonic_v3          threads a multiple of 32 in [32, 1024] (above W the
+ *                       extra waves idle; the reference's historical tests
+ *                       run v3 so, test.cu:319-355)
+ *   bitonic_v4_xor      threads a multiple of 32 in [32, min(W, 1024)]
  *   bitonic_v5_static   threads a power of two in [32, min(W, 1024)]
  *   bitonic_v6_register threads a power of two in [32, min(W, 1024)],
  *                       W / threads <= 16

@@ -55,7 +55,8 @@ FAKE = 1 << 40   # never dereferenced: every case below is rejected first
     ((0, 4, 100, 2, 64, 0), -2),                    # v1 needs threads == W
     ((0, 4, 2000, 3, 0, 0), -2),                    # v2 needs W <= 1024
     ((0, 4, 100, 4, 48, 0), -2),                    # v3 needs a multiple of 32
-    ((0, 4, 100, 4, 256, 0), -2),                   # v3 threads <= W
+    ((0, 4, 100, 4, 1056, 0), -2),                  # v3 threads <= 1024
+    ((0, 4, 100, 5, 256, 0), -2),                   # v4 threads <= W
     ((0, 4, 100, 6, 96, 0), -2),                    # v5 needs a power of two
     ((0, 4, 4096, 7, 128, 0), -2),                  # v6 needs W/threads <= 16
     ((0, 4, 4096, 7, 2048, 0), -2),                 # ... and threads <= 1024
