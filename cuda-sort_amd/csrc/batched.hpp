@@ -151,6 +151,7 @@ CS_DECLARE_TILES(i32)
 CS_DECLARE_TILES(u32)
 CS_DECLARE_TILES(f32)
 CS_DECLARE_TILES(u64)
+CS_DECLARE_TILES(f64)
 CS_DECLARE_TILES(i64)
 CS_DECLARE_TILES(rec32)
 CS_DECLARE_TILES(rec64)
@@ -159,6 +160,7 @@ KernelInfo lds_kernel_i32();
 KernelInfo lds_kernel_u32();
 KernelInfo lds_kernel_f32();
 KernelInfo lds_kernel_u64();
+KernelInfo lds_kernel_f64();
 KernelInfo lds_kernel_i64();
 KernelInfo lds_kernel_rec32();
 KernelInfo lds_kernel_rec64();

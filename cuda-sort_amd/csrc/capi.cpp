@@ -19,6 +19,7 @@ KernelInfo tile_kernel(int dtype, int logw, int loge) {
         CS_TILE_CASE(CS_U32, u32)
         CS_TILE_CASE(CS_F32, f32)
         CS_TILE_CASE(CS_U64, u64)
+        CS_TILE_CASE(CS_F64, f64)
         CS_TILE_CASE(CS_I64, i64)
         CS_TILE_CASE(CS_REC_I32_I32, rec32)
         CS_TILE_CASE(CS_REC_I64_I32, rec64)
@@ -33,6 +34,7 @@ KernelInfo lds_kernel(int dtype) {
         case CS_U32: return lds_kernel_u32();
         case CS_F32: return lds_kernel_f32();
         case CS_U64: return lds_kernel_u64();
+        case CS_F64: return lds_kernel_f64();
         case CS_I64: return lds_kernel_i64();
         case CS_REC_I32_I32: return lds_kernel_rec32();
         case CS_REC_I64_I32: return lds_kernel_rec64();
@@ -42,7 +44,7 @@ KernelInfo lds_kernel(int dtype) {
 
 namespace {
 
-bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_I64; }
+bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_F64; }
 
 int ceil_log2(int64_t n) {
     int l = 0;
@@ -173,7 +175,7 @@ int cs_version(void) { return 100; }
 size_t cs_dtype_size(int dtype) {
     switch (dtype) {
         case CS_I32: case CS_U32: case CS_F32: return 4;
-        case CS_U64: case CS_I64: case CS_REC_I32_I32: return 8;
+        case CS_U64: case CS_I64: case CS_F64: case CS_REC_I32_I32: return 8;
         case CS_REC_I64_I32: return 16;
         default: return 0;
     }

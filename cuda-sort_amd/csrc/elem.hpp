@@ -48,6 +48,15 @@ template <> struct Elem<CS_F32> {
     __device__ static int32_t payload(Storage) { return 0; }
     __device__ static Storage make(Key k, int32_t) { return k ^ (uint32_t(int32_t(~k) >> 31) | 0x80000000u); }
 };
+// fp64 likewise: raw IEEE bits, totalOrder key over uint64.
+template <> struct Elem<CS_F64> {
+    using Storage = uint64_t; using Key = uint64_t;
+    static constexpr bool kPayload = false;
+    static constexpr Key kPad = ~uint64_t(0);
+    __device__ static Key key(Storage b) { return b ^ (uint64_t(int64_t(b) >> 63) | (uint64_t(1) << 63)); }
+    __device__ static int32_t payload(Storage) { return 0; }
+    __device__ static Storage make(Key k, int32_t) { return k ^ (uint64_t(int64_t(~k) >> 63) | (uint64_t(1) << 63)); }
+};
 template <> struct Elem<CS_U64> {
     using Storage = uint64_t; using Key = uint64_t;
     static constexpr bool kPayload = false;

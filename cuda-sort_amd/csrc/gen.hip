@@ -106,6 +106,29 @@ __device__ uint32_t dist_f32(int dist, int64_t n, int64_t g, uint64_t seed) {
     return __float_as_uint(float(dist_i32(dist, n, g, seed)));
 }
 
+// fp64: every bit pattern for UNIFORM; Irwin-Hall(12) - 6 in double plus 1 %
+// specials (NaNs, infinities, -0, +0) for NORMAL_SPECIAL; the integer
+// distributions converted to double otherwise.
+__device__ uint64_t dist_f64(int dist, int64_t n, int64_t g, uint64_t seed) {
+    if (dist == CS_DIST_UNIFORM) return splitmix64(seed ^ uint64_t(g));
+    if (dist == CS_DIST_NORMAL_SPECIAL) {
+        const uint32_t h = u32hash(seed ^ kSaltSpecial, uint64_t(g));
+        if (h % 100u == 0) {
+            switch ((h >> 8) & 7u) {
+                case 0: return 0x7FF8000000000000ull; case 1: return 0xFFF8000000000000ull;
+                case 2: return 0x7FF0000000000001ull; case 3: return 0xFFFFFFFFFFFFFFFFull;
+                case 4: return 0x8000000000000000ull; case 5: return 0x7FF0000000000000ull;
+                case 6: return 0xFFF0000000000000ull; default: return 0ull;
+            }
+        }
+        int64_t sum = 0;
+        for (uint64_t i = 0; i < 12; ++i) sum += int64_t(u32hash(seed ^ kSaltNormal, uint64_t(g) * 16u + i));
+        sum -= int64_t(6) << 32;
+        return uint64_t(__double_as_longlong(double(sum) * 0x1p-32));
+    }
+    return uint64_t(__double_as_longlong(double(dist_i32(dist, n, g, seed))));
+}
+
 __global__ void k_generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, void* out) {
     for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < count; g += int64_t(gridDim.x) * blockDim.x) {
         switch (dtype) {
@@ -114,6 +137,7 @@ __global__ void k_generate(int dtype, int dist, int64_t n, int64_t count, uint64
             case CS_F32: static_cast<uint32_t*>(out)[g] = dist_f32(dist, n, g, seed); break;
             case CS_U64: static_cast<uint64_t*>(out)[g] = dist_u64(dist, n, g, seed); break;
             case CS_I64: static_cast<int64_t*>(out)[g] = int64_t(dist_u64(dist, n, g, seed)); break;
+            case CS_F64: static_cast<uint64_t*>(out)[g] = dist_f64(dist, n, g, seed); break;
             case CS_REC_I32_I32: {
                 int32_t* r = static_cast<int32_t*>(out) + 2 * g;
                 r[0] = dist_i32(dist, n, g, seed);

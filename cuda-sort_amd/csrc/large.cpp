@@ -15,6 +15,7 @@ LargeKernels large_kernels_u32();
 LargeKernels large_kernels_f32();
 LargeKernels large_kernels_u64();
 LargeKernels large_kernels_i64();
+LargeKernels large_kernels_f64();
 LargeKernels large_kernels_rec32();
 LargeKernels large_kernels_rec64();
 
@@ -25,6 +26,7 @@ LargeKernels large_kernels_for(int dtype) {
         case CS_F32: return large_kernels_f32();
         case CS_U64: return large_kernels_u64();
         case CS_I64: return large_kernels_i64();
+        case CS_F64: return large_kernels_f64();
         case CS_REC_I32_I32: return large_kernels_rec32();
         case CS_REC_I64_I32: return large_kernels_rec64();
         default: return LargeKernels{};
@@ -48,7 +50,7 @@ struct cs_workspace {
 
 namespace {
 
-bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_I64; }
+bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_F64; }
 
 // Chunk sort + merge passes.  Result in `final` (== data for in place).
 // `scratch` holds at least n elements and may be `data` itself for an

@@ -89,6 +89,7 @@ PartKernels part_kernels_u32();
 PartKernels part_kernels_f32();
 PartKernels part_kernels_u64();
 PartKernels part_kernels_i64();
+PartKernels part_kernels_f64();
 PartKernels part_kernels_rec32();
 PartKernels part_kernels_rec64();
 
@@ -99,6 +100,7 @@ PartKernels part_kernels_for(int dtype) {
         case CS_F32: return part_kernels_f32();
         case CS_U64: return part_kernels_u64();
         case CS_I64: return part_kernels_i64();
+        case CS_F64: return part_kernels_f64();
         case CS_REC_I32_I32: return part_kernels_rec32();
         case CS_REC_I64_I32: return part_kernels_rec64();
         default: return PartKernels{};

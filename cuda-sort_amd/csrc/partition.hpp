@@ -290,8 +290,6 @@ template <int DT>
 __global__ void __launch_bounds__(kMergeThreads)
 k_seg_merge_pass(const typename Elem<DT>::Storage* src, typename Elem<DT>::Storage* dst, const MergeTile* items,
                  int64_t run, int copy_only) {
-    using E = Elem<DT>;
-    using S = typename E::Storage;
     constexpr int TILE = merge_tile<DT>();
     const MergeTile it = items[blockIdx.x];
     const int tid = int(threadIdx.x);

@@ -23,6 +23,7 @@ SegKernels seg_kernels_u32();
 SegKernels seg_kernels_f32();
 SegKernels seg_kernels_u64();
 SegKernels seg_kernels_i64();
+SegKernels seg_kernels_f64();
 SegKernels seg_kernels_rec32();
 SegKernels seg_kernels_rec64();
 
@@ -33,6 +34,7 @@ SegKernels seg_kernels_for(int dtype) {
         case CS_F32: return seg_kernels_f32();
         case CS_U64: return seg_kernels_u64();
         case CS_I64: return seg_kernels_i64();
+        case CS_F64: return seg_kernels_f64();
         case CS_REC_I32_I32: return seg_kernels_rec32();
         case CS_REC_I64_I32: return seg_kernels_rec64();
         default: return SegKernels{};
@@ -191,7 +193,7 @@ hipError_t update_graph(cs_segplan* p, const void* in, void* out, const int64_t*
 int make_plan_impl(cs_segplan** out, int dtype, const int64_t* h_offsets, int64_t segments, uint32_t flags,
                    const int* force_variant) {
     if (!out || segments < 0 || (segments > 0 && !h_offsets)) return CS_E_INVALID_VALUE;
-    if (dtype < CS_I32 || dtype > CS_I64) return CS_E_INVALID_VALUE;
+    if (dtype < CS_I32 || dtype > CS_F64) return CS_E_INVALID_VALUE;
     if (flags & ~(CS_SEG_NO_GRAPH)) return CS_E_INVALID_VALUE;
     if (segments > (int64_t(1) << 31) - 1) return CS_E_INVALID_VALUE;
     cs_segplan* p = new cs_segplan();
@@ -398,13 +400,13 @@ int cs_segmented_plan_info(const cs_segplan* plan, int32_t* work_items, int32_t*
 }
 
 int32_t cs_segmented_max_len(int dtype) {
-    if (dtype < CS_I32 || dtype > CS_I64) return 0;
+    if (dtype < CS_I32 || dtype > CS_F64) return 0;
     return int32_t(1) << seg_kernels_for(dtype).max_logw;
 }
 
 int cs_segmented_sort_plain(int dtype, const void* in, void* out, const int64_t* d_offsets, int64_t segments,
                             int32_t max_len, hipStream_t stream) {
-    if (dtype < CS_I32 || dtype > CS_I64 || segments < 0 || max_len < 0) return CS_E_INVALID_VALUE;
+    if (dtype < CS_I32 || dtype > CS_F64 || segments < 0 || max_len < 0) return CS_E_INVALID_VALUE;
     if (segments == 0) return CS_OK;
     if (!in || !out || !d_offsets) return CS_E_INVALID_VALUE;
     const SegKernels k = seg_kernels_for(dtype);
@@ -414,13 +416,13 @@ int cs_segmented_sort_plain(int dtype, const void* in, void* out, const int64_t*
 }
 
 int cs_segmented_get_variant(int dtype, int32_t bucket, int32_t* variant) {
-    if (dtype < CS_I32 || dtype > CS_I64 || bucket < 0 || bucket >= kMaxBuckets || !variant) return CS_E_INVALID_VALUE;
+    if (dtype < CS_I32 || dtype > CS_F64 || bucket < 0 || bucket >= kMaxBuckets || !variant) return CS_E_INVALID_VALUE;
     *variant = tuned_variant(dtype, bucket);
     return CS_OK;
 }
 
 int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant) {
-    if (dtype < CS_I32 || dtype > CS_I64 || bucket < 0 || bucket >= kMaxBuckets) return CS_E_INVALID_VALUE;
+    if (dtype < CS_I32 || dtype > CS_F64 || bucket < 0 || bucket >= kMaxBuckets) return CS_E_INVALID_VALUE;
     if (variant != -1 && !seg_kernels_for(dtype).variant_ok(bucket, variant)) return CS_E_UNSUPPORTED;
     set_tuned_variant(dtype, bucket, variant);
     return CS_OK;
@@ -431,7 +433,7 @@ int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant) {
 // the library's generator) through the persistent kernel, keep the fastest
 // (median of `trials` timed runs).  Allocates scratch; not for timed paths.
 int cs_segmented_tune(int dtype, int32_t trials, hipStream_t stream) {
-    if (dtype < CS_I32 || dtype > CS_I64 || trials < 1) return CS_E_INVALID_VALUE;
+    if (dtype < CS_I32 || dtype > CS_F64 || trials < 1) return CS_E_INVALID_VALUE;
     const SegKernels k = seg_kernels_for(dtype);
     const size_t esz = cs_dtype_size(dtype);
     const int64_t budget = int64_t(1) << 22;

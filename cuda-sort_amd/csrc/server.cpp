@@ -110,7 +110,7 @@ struct cs_server {
 
 namespace {
 
-bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_I64; }
+bool valid_dtype(int d) { return d >= CS_I32 && d <= CS_F64; }
 
 // Called with s->mu held; returns the batch in `sl` or false.
 bool form_batch(cs_server* s, Slot& sl) {

@@ -59,6 +59,7 @@ template <class T> struct DType;
 template <> struct DType<int> { static constexpr int value = CS_I32; };
 template <> struct DType<unsigned int> { static constexpr int value = CS_U32; };
 template <> struct DType<float> { static constexpr int value = CS_F32; };
+template <> struct DType<double> { static constexpr int value = CS_F64; };
 template <> struct DType<long> { static constexpr int value = sizeof(long) == 8 ? CS_I64 : CS_I32; };
 template <> struct DType<long long> { static constexpr int value = CS_I64; };
 template <> struct DType<unsigned long> { static constexpr int value = sizeof(unsigned long) == 8 ? CS_U64 : CS_U32; };

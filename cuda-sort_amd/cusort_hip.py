@@ -17,10 +17,10 @@ LIB_PATH = os.path.join(_HERE, "libcusort_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cusort_hip.h")
 
 # cs_dtype
-I32, REC_I32_I32, F32, U64, REC_I64_I32, U32, I64 = range(7)
+I32, REC_I32_I32, F32, U64, REC_I64_I32, U32, I64, F64 = range(8)
 DTYPE_NAMES = {"i32": I32, "rec32": REC_I32_I32, "f32": F32, "u64": U64, "rec64": REC_I64_I32,
-               "u32": U32, "i64": I64}
-DTYPE_SIZE = {I32: 4, REC_I32_I32: 8, F32: 4, U64: 8, REC_I64_I32: 16, U32: 4, I64: 8}
+               "u32": U32, "i64": I64, "f64": F64}
+DTYPE_SIZE = {I32: 4, REC_I32_I32: 8, F32: 4, U64: 8, REC_I64_I32: 16, U32: 4, I64: 8, F64: 8}
 
 # cs_method (bench.cu:129-131 names)
 METHODS = ["auto", "bitonic_warp", "bitonic_v1", "bitonic_v2", "bitonic_v3", "bitonic_v4_xor",
@@ -388,7 +388,7 @@ def offsets_from_lengths(lens):
 def _np_dtype(dtype: int):
     import numpy as np
     return {I32: np.dtype("<i4"), F32: np.dtype("<u4"), U32: np.dtype("<u4"), U64: np.dtype("<u8"),
-            I64: np.dtype("<i8"),
+            I64: np.dtype("<i8"), F64: np.dtype("<u8"),
             REC_I32_I32: np.dtype([("key", "<i4"), ("payload", "<i4")]),
             REC_I64_I32: np.dtype([("key", "<i8"), ("payload", "<i4"), ("pad", "<i4")])}[dtype]
 

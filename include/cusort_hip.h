@@ -40,6 +40,7 @@ extern "C" {
  * CS_U64          uint64 key
  * CS_REC_I64_I32  {int64 key; int32 payload; int32 pad} (16 B, key order)
  * CS_U32, CS_I64  uint32 / int64 keys
+ * CS_F64          fp64 keys (raw bits), IEEE-754 totalOrder like CS_F32
  */
 enum cs_dtype {
     CS_I32 = 0,
@@ -48,7 +49,8 @@ enum cs_dtype {
     CS_U64 = 3,
     CS_REC_I64_I32 = 4,
     CS_U32 = 5,
-    CS_I64 = 6
+    CS_I64 = 6,
+    CS_F64 = 7
 };
 
 /* Batched method names follow bench's --methods list (bench.cu:129-131). */

@@ -118,9 +118,29 @@ uint32_t dist_f32_bits(int dist, int64_t n, int64_t g, uint64_t seed) {
     uint32_t bits; std::memcpy(&bits, &v, 4); return bits;
 }
 
+const uint64_t kSpecialF64[8] = {0x7FF8000000000000ull, 0xFFF8000000000000ull, 0x7FF0000000000001ull,
+                                 0xFFFFFFFFFFFFFFFFull, 0x8000000000000000ull, 0x7FF0000000000000ull,
+                                 0xFFF0000000000000ull, 0ull};   // qNaN, -qNaN, sNaN, -NaN, -0, +inf, -inf, +0
+
+uint64_t dist_f64_bits(int dist, int64_t n, int64_t g, uint64_t seed) {
+    if (dist == ORC_D_UNIFORM) return splitmix64(seed ^ uint64_t(g));   // every bit pattern
+    if (dist == ORC_D_NORMAL_SPECIAL) {
+        const uint32_t h = u32hash(seed ^ kSaltSpecial, uint64_t(g));
+        if (h % 100u == 0) return kSpecialF64[(h >> 8) & 7u];
+        int64_t sum = 0;   // Irwin-Hall(12) - 6, exact in double
+        for (uint64_t i = 0; i < 12; ++i) sum += int64_t(u32hash(seed ^ kSaltNormal, uint64_t(g) * 16u + i));
+        sum -= int64_t(6) << 32;
+        const double z = double(sum) * 0x1p-32;
+        uint64_t bits; std::memcpy(&bits, &z, 8); return bits;
+    }
+    const double v = double(dist_i32(dist, n, g, seed));
+    uint64_t bits; std::memcpy(&bits, &v, 8); return bits;
+}
+
 // ------------------------------------------------------------ key orders --
 inline uint32_t f32_ordered(uint32_t b) { return b ^ ((b >> 31) ? 0xFFFFFFFFu : 0x80000000u); }
 inline uint32_t f32_unordered(uint32_t k) { return k ^ ((k >> 31) ? 0x80000000u : 0xFFFFFFFFu); }
+inline uint64_t f64_ordered(uint64_t b) { return b ^ ((b >> 63) ? ~uint64_t(0) : (uint64_t(1) << 63)); }
 
 template <int D> struct Traits;
 template <> struct Traits<ORC_I32> {
@@ -136,6 +156,11 @@ template <> struct Traits<ORC_U32> {
 template <> struct Traits<ORC_F32> {
     using T = uint32_t; using K = uint32_t;   // raw IEEE bits
     static K key(const T& v) { return f32_ordered(v); }
+    static bool same(const T& a, const T& b) { return a == b; }
+};
+template <> struct Traits<ORC_F64> {
+    using T = uint64_t; using K = uint64_t;   // raw IEEE bits
+    static K key(const T& v) { return f64_ordered(v); }
     static bool same(const T& a, const T& b) { return a == b; }
 };
 template <> struct Traits<ORC_U64> {
@@ -175,11 +200,12 @@ template <typename F> void dispatch(int dtype, F&& f) {
         case ORC_REC_I64_I32: f(std::integral_constant<int, ORC_REC_I64_I32>{}); break;
         case ORC_U32: f(std::integral_constant<int, ORC_U32>{}); break;
         case ORC_I64: f(std::integral_constant<int, ORC_I64>{}); break;
+        case ORC_F64: f(std::integral_constant<int, ORC_F64>{}); break;
         default: break;
     }
 }
 
-bool valid_dtype(int d) { return d >= ORC_I32 && d <= ORC_I64; }
+bool valid_dtype(int d) { return d >= ORC_I32 && d <= ORC_F64; }
 
 int host_cores() {
     cpu_set_t set;
@@ -307,6 +333,7 @@ int orc_generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, v
             case ORC_F32: static_cast<uint32_t*>(out)[g] = dist_f32_bits(dist, n, g, seed); break;
             case ORC_U64: static_cast<uint64_t*>(out)[g] = dist_u64(dist, n, g, seed); break;
             case ORC_I64: static_cast<int64_t*>(out)[g] = int64_t(dist_u64(dist, n, g, seed)); break;
+            case ORC_F64: static_cast<uint64_t*>(out)[g] = dist_f64_bits(dist, n, g, seed); break;
             case ORC_REC_I32_I32:
                 static_cast<RecI32*>(out)[g] = RecI32{dist_i32(dist, n, g, seed), int32_t(g)};   // bench.cu:189
                 break;

@@ -52,7 +52,8 @@ enum {
     ORC_U64 = 3,        /* uint64 key */
     ORC_REC_I64_I32 = 4,/* {int64 key, int32 payload, int32 pad}, 16 B */
     ORC_U32 = 5,        /* uint32 key */
-    ORC_I64 = 6         /* int64 key */
+    ORC_I64 = 6,        /* int64 key */
+    ORC_F64 = 7         /* fp64 key, IEEE-754 totalOrder */
 };
 
 /* Distributions.  0-8 are the nine seeded distributions of the batched and

@@ -20,7 +20,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-I32, REC_I32_I32, F32, U64, REC_I64_I32, U32, I64 = range(7)
+I32, REC_I32_I32, F32, U64, REC_I64_I32, U32, I64, F64 = range(8)
 
 (D_UNIFORM, D_SORTED, D_REVERSED, D_CONSTANT, D_DISTINCT16, D_PERTURBED, D_ORGAN_PIPE,
  D_ALT_EXTREMES, D_DUP50, D_EXTREMES6, D_RANDOM1M, D_BINARY, D_NORMAL_SPECIAL,
@@ -38,7 +38,7 @@ REC_I64 = np.dtype([("key", "<i8"), ("payload", "<i4"), ("pad", "<i4")])
 
 NP_DTYPE = {I32: np.dtype("<i4"), REC_I32_I32: REC_I32, F32: np.dtype("<u4"),
             U64: np.dtype("<u8"), REC_I64_I32: REC_I64, U32: np.dtype("<u4"),
-            I64: np.dtype("<i8")}
+            I64: np.dtype("<i8"), F64: np.dtype("<u8")}
 
 _lib = None
 
@@ -165,6 +165,18 @@ def sort_keys_numpy(dtype: int, data: np.ndarray, n: int) -> np.ndarray:
         keys = np.where(b >> 31 == 1, b ^ np.uint32(0xFFFFFFFF), b ^ np.uint32(0x80000000)).reshape(-1, n)
         s = np.sort(keys, axis=1, kind="stable").reshape(-1)
         return np.where(s >> 31 == 1, s ^ np.uint32(0x80000000), s ^ np.uint32(0xFFFFFFFF)).astype(np.uint32)
+    elif dtype == F64:
+        # totalOrder from the float value itself: NaNs by sign to the ends
+        # (payload order among them from the bit pattern), -0 before +0.
+        v = data.view(np.float64).reshape(-1, n)
+        b = data.reshape(-1, n)
+        nan = np.isnan(v)
+        neg = np.signbit(v)
+        cls = np.where(nan & neg, 0, np.where(nan, 2, 1))
+        tie = np.where(nan & neg, ~b, np.where(nan, b, neg.astype(np.uint64) ^ np.uint64(1)))
+        val = np.where(nan, 0.0, v)
+        order = np.lexsort((tie, val, cls), axis=1)
+        return np.take_along_axis(b, order, axis=1).reshape(-1)
     else:
         keys = data.reshape(-1, n)
     return np.sort(keys, axis=1, kind="stable").reshape(-1)

@@ -106,6 +106,17 @@ int main() {
         expect_sorted_equal(h, download(d, h.size()), n, "HybridSortPartitionedV3 u64");
         (void)hipFree(d);
     }
+    {   // fp64 keys (finite values: operator< agrees with totalOrder up to -0 == +0)
+        std::normal_distribution<double> normal(0.0, 1e3);
+        for (int n : {1000, 200000}) {
+            std::vector<double> h(static_cast<size_t>(n));
+            for (auto& v : h) v = normal(rng);
+            double* d = upload(h);
+            cusort_hip::Sort<double>(d, n);
+            expect_sorted_equal(h, download(d, h.size()), n, "Sort double");
+            (void)hipFree(d);
+        }
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }

@@ -54,14 +54,14 @@ def test_auto_i32_sizes(cs, orc, gpu, n):
         run_case(cs, orc, gpu, cs.I32, dist, n, arrays_for(n))
 
 
-@pytest.mark.parametrize("dtype", ["I32", "REC_I32_I32", "F32", "U64", "REC_I64_I32", "U32", "I64"])
+@pytest.mark.parametrize("dtype", ["I32", "REC_I32_I32", "F32", "U64", "REC_I64_I32", "U32", "I64", "F64"])
 @pytest.mark.parametrize("n", [1, 5, 32, 33, 129, 1000, 2048, 4096])
 def test_auto_dtypes_nine_distributions(cs, orc, gpu, dtype, n):
     dt = getattr(cs, dtype)
     if n > cs.batched_max_n(dt):
         pytest.skip("beyond capacity")
     dists = orc.NINE + [orc.D_EXTREMES6, orc.D_RANDOM1M, orc.D_BINARY]
-    if dt == cs.F32:
+    if dt in (cs.F32, cs.F64):
         dists = dists + [orc.D_NORMAL_SPECIAL]
     for dist in dists:
         run_case(cs, orc, gpu, dt, dist, n, arrays_for(n, 1 << 16), seed=1000 + dist)

@@ -30,7 +30,7 @@ def test_strings_and_sizes(cs):
     for name, idx in zip(cs.METHODS, range(8)):
         assert L.cs_method_from_name(name.encode()) == idx
     assert L.cs_method_from_name(b"cub_block_merge") == -1
-    assert [L.cs_dtype_size(d) for d in range(7)] == [4, 8, 4, 8, 16, 4, 8]
+    assert [L.cs_dtype_size(d) for d in range(8)] == [4, 8, 4, 8, 16, 4, 8, 8]
     assert cs.batched_max_n(cs.I32) == 16384 and cs.batched_max_n(cs.REC_I32_I32) == 8192
     assert cs.batched_max_n(cs.REC_I64_I32) == 4096 and cs.batched_max_n(99) == 0
     assert "domain" in cs.strerror(cs.E_UNSUPPORTED)

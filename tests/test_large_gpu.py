@@ -40,7 +40,7 @@ DISTS = ["D_UNIFORM", "D_SORTED", "D_REVERSED", "D_CONSTANT", "D_DUP50", "D_EXTR
 
 
 @pytest.mark.parametrize("method", ["merge", "merge_out", "merge_ws", "partition", "hybrid"])
-@pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32"])
+@pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32", "F64"])
 def test_large_methods(cs, orc, gpu, dtype, method):
     dt = getattr(cs, dtype)
     for n in SIZES:

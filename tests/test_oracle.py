@@ -79,7 +79,7 @@ def test_uniform_and_dup50_statistics(orc):
     assert 0.005 < special.mean() < 0.012
 
 
-@pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32"])
+@pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32", "F64"])
 def test_sort_agrees_with_numpy(orc, dtype):
     dt = getattr(orc, dtype)
     for dist in orc.NINE + [orc.D_EXTREMES6, orc.D_RANDOM1M]:
@@ -117,6 +117,34 @@ def test_total_order_special_values(orc):
     data = np.array(order[::-1] * 3, dtype=np.uint32)
     got = orc.sort_batched(orc.F32, data, data.size)
     assert [int(x) for x in got[::3]] == order
+
+
+def test_total_order_special_values_f64(orc):
+    # IEEE-754 totalOrder: -NaN (larger payload first) < -inf < negatives <
+    # -0 < +0 < positives < +inf < +NaN (smaller payload first)
+    order = [0xFFFFFFFFFFFFFFFF, 0xFFF8000000000000, 0xFFF0000000000001, 0xFFF0000000000000,
+             0xBFF0000000000000, 0x8000000000000001, 0x8000000000000000, 0x0000000000000000,
+             0x0000000000000001, 0x3FF0000000000000, 0x7FEFFFFFFFFFFFFF, 0x7FF0000000000000,
+             0x7FF0000000000001, 0x7FF8000000000000]
+    data = np.array(order[::-1] * 3, dtype=np.uint64)
+    got = orc.sort_batched(orc.F64, data, data.size)
+    assert [int(x) for x in got[::3]] == order
+    # sorted N(0,1)+specials: non-NaN values ascend as doubles, NaNs sit at the ends by sign
+    n = 1 << 12
+    d = orc.generate(orc.F64, orc.D_NORMAL_SPECIAL, n, n * 8, seed=11)
+    s = orc.sort_batched(orc.F64, d, n).reshape(-1, n)
+    for row in s:
+        v = row.view(np.float64)
+        nan = np.isnan(v)
+        neg_nan = nan & np.signbit(v)
+        k = int(neg_nan.sum())
+        assert neg_nan[:k].all()
+        body = v[k:k + int((~nan).sum())]
+        assert not np.isnan(body).any() and (body[1:] >= body[:-1]).all()
+        assert nan[k + body.size:].all() and not np.signbit(v[k + body.size:]).any()
+        zero_sign = np.signbit(body[body == 0]).astype(np.int8)
+        assert (np.diff(zero_sign) <= 0).all()   # -0 before +0
+    assert orc.validate_batched(orc.F64, d, s.reshape(-1), n)[0] == 0
 
 
 def test_validator_catches_planted_mistakes(orc):

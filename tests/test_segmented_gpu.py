@@ -36,7 +36,7 @@ def random_offsets(rng, segs, max_len):
     return np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
 
 
-@pytest.mark.parametrize("dtype", ["I32", "F32", "REC_I32_I32", "U64", "REC_I64_I32", "U32", "I64"])
+@pytest.mark.parametrize("dtype", ["I32", "F32", "REC_I32_I32", "U64", "REC_I64_I32", "U32", "I64", "F64"])
 def test_random_ragged_all_variants(cs, orc, gpu, dtype):
     dt = getattr(cs, dtype)
     rng = np.random.default_rng(7 + dt)
