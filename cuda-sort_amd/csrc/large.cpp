@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <mutex>
 
 #include "../../include/cusort_hip.h"
 #include "gen.hpp"
@@ -18,6 +20,39 @@ LargeKernels large_kernels_i64();
 LargeKernels large_kernels_f64();
 LargeKernels large_kernels_rec32();
 LargeKernels large_kernels_rec64();
+
+hipError_t pool_malloc_async(void** p, size_t bytes, hipStream_t s) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static hipMemPool_t pools[kMaxDevices] = {};
+    static bool tried[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipMemPool_t pool = nullptr;
+    if (dev >= 0 && dev < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!tried[dev]) {
+            tried[dev] = true;
+            hipMemPoolProps props = {};
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            hipMemPool_t created = nullptr;
+            if (hipMemPoolCreate(&created, &props) == hipSuccess) {
+                uint64_t keep = UINT64_MAX;
+                if (hipMemPoolSetAttribute(created, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess)
+                    pools[dev] = created;
+                else
+                    (void)hipMemPoolDestroy(created);
+            }
+            (void)hipGetLastError();
+        }
+        pool = pools[dev];
+    }
+    // Without a pool of its own the device's default pool serves.
+    return pool ? hipMallocFromPoolAsync(p, bytes, pool, s) : hipMallocAsync(p, bytes, s);
+}
 
 LargeKernels large_kernels_for(int dtype) {
     switch (dtype) {
@@ -66,7 +101,7 @@ int merge_sort(int dtype, void* data, int64_t n, void* final, void* scratch, hip
     // Start co-rank of every merge tile (one int64 each), stream ordered.
     const int64_t tiles = (n + k.tile - 1) / k.tile;
     int64_t* cor = nullptr;
-    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&cor), size_t(tiles) * sizeof(int64_t), stream);
+    hipError_t e = pool_malloc_async(reinterpret_cast<void**>(&cor), size_t(tiles) * sizeof(int64_t), stream);
     if (e != hipSuccess) return int(e);
     // The chunk sort lands where an even number of ping-pong passes ends in `final`.
     void* a = (passes % 2 == 0) ? final : scratch;
@@ -148,7 +183,7 @@ int cs_sort_merge(int dtype, void* data, int64_t n, int32_t tile, int32_t merge_
     if (ext_buf && ext_elems >= size_t(n)) return merge_sort(dtype, data, n, data, ext_buf, stream);
     // No usable buffer: a stream-ordered temporary (cudaMallocAsync in the reference).
     void* tmp = nullptr;
-    hipError_t e = hipMallocAsync(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
+    hipError_t e = pool_malloc_async(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
     if (e != hipSuccess) return int(e);
     const int rc = merge_sort(dtype, data, n, data, tmp, stream);
     e = hipFreeAsync(tmp, stream);
@@ -170,7 +205,7 @@ int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32
     if (n <= 1) return CS_OK;
     if (!data) return CS_E_INVALID_VALUE;
     void* tmp = nullptr;
-    hipError_t e = hipMallocAsync(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
+    hipError_t e = pool_malloc_async(&tmp, size_t(n) * cs_dtype_size(dtype), stream);
     if (e != hipSuccess) return int(e);
     const int rc = partition_sort(dtype, data, n, data, tmp, size_t(n), min_parts, max_parts, stream);
     e = hipFreeAsync(tmp, stream);
@@ -188,7 +223,7 @@ int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream
     if (ws && ws->capacity >= size_t(n)) {
         scratch = ws->values;
     } else {
-        const hipError_t e = hipMallocAsync(&scratch, size_t(n) * cs_dtype_size(dtype), stream);
+        const hipError_t e = pool_malloc_async(&scratch, size_t(n) * cs_dtype_size(dtype), stream);
         if (e != hipSuccess) return int(e);
         owned = true;
     }

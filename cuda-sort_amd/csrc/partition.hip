@@ -3,10 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/cusort_hip.h"
-#include "partition.hpp"
+#include "gen.hpp"
+#include "ragged.hpp"
 
 namespace cs {
 
@@ -78,10 +80,10 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(int64_t* v, int64_t
     }
 }
 
-__global__ void k_bucket_totals(const int64_t* offsets, int nb, int blocks, int64_t n, int64_t* bucket_off) {
-    const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (b < nb) bucket_off[b] = offsets[int64_t(b) * blocks];
-    if (b == nb) bucket_off[b] = n;
+__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts) {
+    const PartSeg g = segs[blockIdx.x];
+    for (int b = int(threadIdx.x); b < g.nb; b += int(blockDim.x))
+        starts[g.first + b] = offsets[g.cell0 + int64_t(b) * g.stride];
 }
 
 PartKernels part_kernels_i32();
@@ -122,15 +124,112 @@ struct Scoped {   // stream-ordered temporary
     T* p = nullptr;
     hipStream_t s;
     explicit Scoped(hipStream_t st) : s(st) {}
-    hipError_t alloc(size_t count) { return hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s); }
+    hipError_t alloc(size_t count) { return pool_malloc_async(reinterpret_cast<void**>(&p), count * sizeof(T), s); }
     ~Scoped() { if (p) (void)hipFreeAsync(p, s); }
 };
 
 }  // namespace
 
+namespace {
+
+constexpr int64_t kPartBlockElems = 65536;   // elements per workgroup of count / scatter
+constexpr int kPartOneLevel = 256;           // parts up to which one level is used
+
+// One partition level: the workgroups of every segment, the segments' bucket
+// start descriptors, and per segment its end and whether its buckets are
+// equality buckets by construction (a first-level equality bucket).
+struct Level {
+    std::vector<PartBlock> work;
+    std::vector<PartSeg> segs;
+    std::vector<int64_t> seg_end;
+    std::vector<char> seg_equal;
+    int64_t cells = 0;
+    int64_t nbuckets = 0;
+    int max_ns = 0;
+};
+
+void clear_level(Level& L) {
+    L.work.clear();
+    L.segs.clear();
+    L.seg_end.clear();
+    L.seg_equal.clear();
+    L.cells = 0;
+    L.nbuckets = 0;
+    L.max_ns = 0;
+}
+
+// Host arrays of one call, kept per thread so that their pages stay mapped
+// from call to call (first-touch page faults of fresh multi-MB vectors cost
+// more than the loops that fill them).
+struct PartHost {
+    Level l1, l2;
+    std::vector<int64_t> st1, st2, seg;
+    std::vector<MergeTile> tiles;
+    std::vector<int> list;
+    std::vector<SegItem> items;
+};
+
+void add_segment(Level& L, int64_t begin, int64_t end, int sp0, int ns, int lo, int hi, bool equal) {
+    const int64_t len = end - begin;
+    if (len <= 0) return;
+    const int64_t blocks = (len + kPartBlockElems - 1) / kPartBlockElems;
+    const int nb = 2 * ns + 1;
+    L.segs.push_back(PartSeg{L.cells, L.nbuckets, int32_t(blocks), nb});
+    L.seg_end.push_back(end);
+    L.seg_equal.push_back(equal ? 1 : 0);
+    for (int64_t k = 0; k < blocks; ++k) {
+        const int64_t b0 = begin + k * kPartBlockElems;
+        L.work.push_back(PartBlock{b0, std::min(end, b0 + kPartBlockElems), L.cells + k, int32_t(blocks), sp0, ns,
+                                   lo, hi, 0});
+    }
+    L.cells += int64_t(nb) * blocks;
+    L.nbuckets += nb;
+    L.max_ns = std::max(L.max_ns, ns);
+}
+
+// Count, scan and scatter of one level (src -> dst), then the bucket starts
+// read back into `starts` (one host synchronisation).
+hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, const void* splitters, void* dst,
+                     std::vector<int64_t>& starts, hipStream_t stream) {
+    Scoped<PartBlock> work(stream);
+    Scoped<PartSeg> segs(stream);
+    Scoped<int64_t> counts(stream), sums(stream), d_starts(stream);
+    hipError_t e = work.alloc(L.work.size());
+    if (e == hipSuccess) e = segs.alloc(L.segs.size());
+    if (e == hipSuccess) e = counts.alloc(size_t(L.cells));
+    if (e == hipSuccess) e = sums.alloc(size_t((L.cells + kScanTile - 1) / kScanTile) + 1);
+    if (e == hipSuccess) e = d_starts.alloc(size_t(L.nbuckets));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(work.p, L.work.data(), L.work.size() * sizeof(PartBlock), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(segs.p, L.segs.data(), L.segs.size() * sizeof(PartSeg), hipMemcpyHostToDevice, stream);
+    const int blocks = int(L.work.size());
+    if (e == hipSuccess) e = pk.count(src, splitters, work.p, blocks, counts.p, stream);
+    if (e == hipSuccess) e = exclusive_scan(counts.p, L.cells, sums.p, stream);
+    if (e == hipSuccess) e = pk.scatter(src, splitters, work.p, blocks, L.max_ns, counts.p, dst, stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_part_starts, dim3(unsigned(L.segs.size())), dim3(256), 0, stream, segs.p, counts.p,
+                           d_starts.p);
+        e = hipGetLastError();
+    }
+    starts.assign(size_t(L.nbuckets), 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(starts.data(), d_starts.p, starts.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+}
+
+}  // namespace
+
 // Sorts data[0, n) into out (out may equal data) with tmp as scratch of at
-// least n elements.  Two host synchronisations: bucket sizes after the
-// scatter, and before the one-shot segment plan is released.
+// least n elements.  Sample sort into buckets of ~3/8 of the on-chip tile:
+// one partition level (data -> tmp) for up to kPartOneLevel parts, else two
+// (data -> tmp -> out) with p1 x p2 parts taken from one global splitter
+// list, so the second level needs no sampling of its own.  The buckets are
+// then sorted as one ragged batch (tmp -> out, or in place in out); the
+// rare bucket longer than a tile is cut into chunks merged by tiled passes.
+// Host synchronisations: bucket starts after each level, and before the
+// one-shot segment plan is released.
 int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_t tmp_elems, int min_parts,
                    int max_parts, hipStream_t stream) {
     const LargeKernels lk = large_kernels_for(dtype);
@@ -139,85 +238,135 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
     if (tmp_elems < size_t(n)) return CS_E_WORKSPACE;
     if (n <= lk.chunk) return cs_batched_sort(dtype, data, out, 1, int32_t(n), CS_METHOD_AUTO, 0, 0, stream);
 
-    // Default: buckets of ~2^18 keys when the chunk is 2^13 keys (4-byte
-    // elements), ~2^17 when it is 2^12.  The largest bucket then lands just
-    // above a power of two and needs an even number of merge passes (6), so
-    // the result ends where it started and no copy-back pass is needed; more
-    // parts fragment the scatter's runs.  At most 1024 parts.  Measured on
-    // MI355X, 2^28 int32: 8.0 ms at 1024 parts, 8.8 at 512 (7 passes + a
-    // copy), 10.6 at 2048; 2^27 uint64: 9.1 ms at 1024, 9.7 at 256.
-    const int bucket_log2 = lk.chunk == 8192 ? 18 : 17;
-    int parts = int(std::min<int64_t>(1024, std::max<int64_t>(2, n >> bucket_log2)));
-    if (min_parts > 0) parts = std::max(parts, min_parts);   // the caller's MinNbParts / MaxNbParts
-    if (max_parts > 0) parts = std::min(parts, max_parts);
-    parts = std::max(2, std::min(parts, kPartMaxSplitters + 1));
-    const int ns = parts - 1;
-    const int nb = 2 * ns + 1;
-    const int m = parts * 64;   // oversampling: 64 sample keys per part
-    const int blocks = int(std::min<int64_t>(1024, std::max<int64_t>(1, (n + 65535) / 65536)));
-    const int64_t per_block = (n + blocks - 1) / blocks;
-    const int64_t cells = int64_t(nb) * blocks;
+    // Buckets average 3/8 of the tile (3072 4-byte / 1536 wider keys): with
+    // 32-64 samples per part the largest of ~10^5 buckets stays under the
+    // tile and most fill the next power of two to ~75 %.
+    const int64_t target = lk.chunk * 3 / 8;
+    int64_t P = (n + target - 1) / target;
+    if (min_parts > 0) P = std::max<int64_t>(P, min_parts);   // the caller's MinNbParts / MaxNbParts
+    if (max_parts > 0) P = std::min<int64_t>(P, max_parts);
+    const int64_t pmax = int64_t(kPartMaxSplitters + 1) * (kPartMaxSplitters + 1);
+    P = std::max<int64_t>(2, std::min(P, pmax));
+    int p1 = int(P), p2 = 1;
+    if (P > kPartOneLevel) {
+        p1 = int(std::ceil(std::sqrt(double(P))));
+        p2 = int(P / p1);   // p1 * p2 <= P: a caller's part bound holds
+        p1 = std::min(p1, kPartMaxSplitters + 1);
+        p2 = std::min(p2, kPartMaxSplitters + 1);
+    }
+    const int64_t parts = int64_t(p1) * p2;
+    const int64_t m = parts * (p2 > 1 ? 32 : 64);   // oversampling
 
-    Scoped<unsigned char> samples(stream), samples_tmp(stream), splitters(stream);
-    Scoped<int64_t> counts(stream), sums(stream), bucket_off(stream);
+    Scoped<unsigned char> samples(stream), samples_tmp(stream), gsp(stream), l1sp(stream);
     hipError_t e = samples.alloc(size_t(m) * esz);
     if (e == hipSuccess) e = samples_tmp.alloc(size_t(m) * esz);
-    if (e == hipSuccess) e = splitters.alloc(size_t(std::max(ns, 1)) * size_t(pk.key_size));
-    if (e == hipSuccess) e = counts.alloc(size_t(cells));
-    if (e == hipSuccess) e = sums.alloc(size_t((cells + kScanTile - 1) / kScanTile) + 1);
-    if (e == hipSuccess) e = bucket_off.alloc(size_t(nb) + 1);
+    if (e == hipSuccess) e = gsp.alloc(size_t(parts - 1) * size_t(pk.key_size));
+    if (e == hipSuccess && p2 > 1) e = l1sp.alloc(size_t(std::max(p1 - 1, 1)) * size_t(pk.key_size));
     if (e != hipSuccess) return int(e);
 
-    // 1-2. splitters from a sorted hashed sample
-    e = pk.sample(data, n, samples.p, m, stream);
+    // splitters from a sorted hashed sample; the first level's are every
+    // p2-th of them
+    e = pk.sample(data, n, samples.p, int(m), stream);
     if (e != hipSuccess) return int(e);
     int rc = cs_sort_merge(dtype, samples.p, m, 0, 0, samples_tmp.p, size_t(m), 1, stream);
     if (rc != CS_OK) return rc;
-    e = pk.splitters(samples.p, m, parts, splitters.p, stream);
-    // 3-5. count, scan, scatter (data -> tmp)
-    if (e == hipSuccess) e = pk.count(data, n, splitters.p, ns, per_block, blocks, counts.p, stream);
-    if (e == hipSuccess) e = exclusive_scan(counts.p, cells, sums.p, stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bucket_totals, dim3(unsigned((nb + 1 + 255) / 256)), dim3(256), 0, stream, counts.p, nb,
-                           blocks, n, bucket_off.p);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = pk.scatter(data, n, splitters.p, ns, per_block, blocks, counts.p, tmp, stream);
-    std::vector<int64_t> boff(size_t(nb) + 1);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(boff.data(), bucket_off.p, boff.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    e = pk.splitters(samples.p, int(m), int(parts), gsp.p, stream);
+    if (e == hipSuccess && p2 > 1) e = pk.gather(gsp.p, p1, p2, l1sp.p, stream);
     if (e != hipSuccess) return int(e);
 
-    // 6. one ragged batch over all buckets; long buckets cut into chunks
+    // level 1: data -> tmp
+    thread_local PartHost host;
+    Level& L1 = host.l1;
+    Level& L2 = host.l2;
+    clear_level(L1);
+    clear_level(L2);
+    add_segment(L1, 0, n, 0, p1 - 1, -1, -1, false);
+    std::vector<int64_t>& st1 = host.st1;
+    e = run_level(pk, L1, data, p2 > 1 ? l1sp.p : gsp.p, tmp, st1, stream);
+    if (e != hipSuccess) return int(e);
+
+    // level 2: tmp -> out, one segment per first-level bucket; equality
+    // buckets and buckets already small are copied through
+    const Level* fin = &L1;
+    const std::vector<int64_t>* fst = &st1;
+    std::vector<int64_t>& st2 = host.st2;
+    void* src = tmp;
+    if (p2 > 1) {
+        const int nb1 = 2 * p1 - 1;
+        for (int b = 0; b < nb1; ++b) {
+            const int64_t b0 = st1[size_t(b)], b1 = b + 1 < nb1 ? st1[size_t(b) + 1] : n;
+            const int j = b / 2;
+            if ((b & 1) || b1 - b0 <= target) {
+                add_segment(L2, b0, b1, 0, 0, -1, -1, (b & 1) != 0);
+            } else {
+                add_segment(L2, b0, b1, j * p2, p2 - 1, j > 0 ? j * p2 - 1 : -1,
+                            j < p1 - 1 ? (j + 1) * p2 - 1 : -1, false);
+            }
+        }
+        e = run_level(pk, L2, tmp, gsp.p, out, st2, stream);
+        if (e != hipSuccess) return int(e);
+        fin = &L2;
+        fst = &st2;
+        src = out;
+    }
+
+    // one ragged batch over all buckets (src -> out); long buckets cut into
+    // chunks.  The level's bucket starts run through the array in order.
     const int64_t chunk = lk.chunk;
-    std::vector<int64_t> seg{0};
-    std::vector<MergeTile> tiles;
+    const std::vector<int64_t>& st = *fst;
+    std::vector<int64_t>& seg = host.seg;
+    seg.assign(1, 0);
+    seg.reserve(st.size() / 2 + size_t(n / chunk) + 2);
+    std::vector<MergeTile>& tiles = host.tiles;
+    tiles.clear();
     int passes = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int64_t b0 = boff[size_t(b)], len = boff[size_t(b) + 1] - b0;
+    size_t g = 0;   // segment of bucket i (only needed for long buckets)
+    for (size_t i = 0; i < st.size(); ++i) {
+        const int64_t b0 = st[i], b1 = i + 1 < st.size() ? st[i + 1] : n;
+        const int64_t len = b1 - b0;
         if (len <= 0) continue;
+        if (b0 != seg.back()) return CS_E_NO_DEVICE;   // buckets must tile the array in order
+        if (len <= chunk) {
+            seg.push_back(b1);
+            continue;
+        }
         for (int64_t c = 0; c < len; c += chunk) seg.push_back(b0 + std::min(len, c + chunk));
-        if (len > chunk && (b % 2) == 0) {   // equality buckets (odd) are sorted once their chunks are
+        while (g + 1 < fin->segs.size() && fin->segs[g + 1].first <= int64_t(i)) ++g;
+        const bool equal = fin->seg_equal[g] || ((int64_t(i) - fin->segs[g].first) & 1);
+        if (!equal) {   // equality buckets are sorted once their chunks are
             int p = 0;
             for (int64_t run = chunk; run < len; run *= 2) ++p;
             passes = std::max(passes, p);
             for (int64_t d0 = 0; d0 < len; d0 += lk.tile) tiles.push_back(MergeTile{b0, len, d0});
         }
     }
-    cs_segplan* plan = nullptr;
-    rc = cs_segmented_plan_create(&plan, dtype, seg.data(), int64_t(seg.size()) - 1, CS_SEG_NO_GRAPH);
+    if (seg.back() != n) return CS_E_NO_DEVICE;
+    std::vector<int>& list = host.list;
+    std::vector<SegItem>& items = host.items;
+    const int64_t nseg = int64_t(seg.size()) - 1;
+    rc = seg_build_items(dtype, seg.data(), nseg, nullptr, list, items, nullptr, nullptr, nullptr);
     if (rc != CS_OK) return rc;
+    const SegShape shape = seg_shape(dtype, items);
     Scoped<int64_t> d_seg(stream);
+    Scoped<int> d_list(stream), d_counters(stream);
+    Scoped<SegItem> d_items(stream);
     Scoped<MergeTile> d_tiles(stream);
     e = d_seg.alloc(seg.size());
-    if (e == hipSuccess) e = hipMemcpyAsync(d_seg.p, seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = d_list.alloc(list.size());
+    if (e == hipSuccess) e = d_items.alloc(items.size());
+    if (e == hipSuccess) e = d_counters.alloc(2);
     if (e == hipSuccess && !tiles.empty()) e = d_tiles.alloc(tiles.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(d_seg.p, seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_list.p, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(SegItem), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess && !tiles.empty())
         e = hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(MergeTile), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) rc = cs_segmented_sort(plan, tmp, out, d_seg.p, stream);
-    else rc = int(e);
-    // 7. merge passes over the long buckets (out <-> tmp), then final placement
+    if (e == hipSuccess)
+        e = seg_launch_all(dtype, shape, src, out, d_seg.p, d_list.p, d_items.p, int(items.size()), d_counters.p,
+                           stream);
+    rc = e == hipSuccess ? CS_OK : int(e);
+    // merge passes over the long buckets (out <-> tmp), then final placement
     void* a = out;
     void* b2 = tmp;
     for (int p = 0; rc == CS_OK && p < passes; ++p) {
@@ -229,8 +378,7 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
         e = pk.seg_merge(a, out, d_tiles.p, int(tiles.size()), 0, 1, stream);
         if (e != hipSuccess) rc = int(e);
     }
-    const hipError_t se = hipStreamSynchronize(stream);   // the plan's buffers are in use until here
-    cs_segmented_plan_destroy(plan);
+    const hipError_t se = hipStreamSynchronize(stream);   // host arrays above are read by the copies
     if (rc == CS_OK && se != hipSuccess) rc = int(se);
     return rc;
 }

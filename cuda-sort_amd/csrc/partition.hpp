@@ -59,34 +59,98 @@ __global__ void k_part_splitters(const typename Elem<DT>::Storage* sorted_sample
 // Elements per thread per tile of the count and scatter kernels.
 template <int DT> constexpr int part_items() { return sizeof(typename Elem<DT>::Storage) == 4 ? 16 : 8; }
 template <int DT> constexpr int part_tile() { return part_items<DT>() * kPartThreads; }
+// The scatter stages larger tiles (16384 4-byte / 8192 8-byte / 4096 16-byte
+// elements) with a wider workgroup: each bucket's run per tile is then long
+// enough that its writes fill whole cache lines.
+template <int DT> constexpr int part_scatter_threads() {
+    return sizeof(typename Elem<DT>::Storage) >= 16 ? 512 : 1024;
+}
+template <int DT> constexpr int part_scatter_tile() { return part_items<DT>() * part_scatter_threads<DT>(); }
 
-// Classification.  A lookup table indexed by the top kPartLutBits bits of
-// the key's order-preserving unsigned image narrows the splitter search:
+// Work of one workgroup of the count and scatter kernels: the element range
+// [begin, end) of one segment, classified against the splitters
+// sp[sp0, sp0 + ns).  The segment's count matrix is bucket-major with one
+// column per workgroup of the segment: (bucket b, this workgroup) is cell
+// `cell + b * stride`.  lo / hi index the splitters that bound every key of
+// the segment from below / above (-1: unbounded); they only narrow the
+// lookup table.  A one-level partition is a single segment over the whole
+// array; the second level runs one segment per first-level bucket.
+struct PartBlock {
+    int64_t begin;
+    int64_t end;
+    int64_t cell;
+    int32_t stride;
+    int32_t sp0;
+    int32_t ns;
+    int32_t lo;
+    int32_t hi;
+    int32_t pad;
+};
+
+// Bucket start positions of a segment: for bucket b of the segment,
+// starts[first + b] = scanned count of cell (cell0 + b * stride).
+struct PartSeg {
+    int64_t cell0;
+    int64_t first;
+    int32_t stride;
+    int32_t nb;
+};
+
+// Classification.  A lookup table indexed by kPartLutBits bits of the key's
+// order-preserving unsigned image, taken relative to the segment's lower
+// bound and scaled to its width, narrows the splitter search:
 // lut[x] = number of splitters whose prefix is below x, so the splitters
 // that can tie with or bracket a key of prefix x are sp[lut[x] .. lut[x+1]).
-// For spread-out keys that range holds zero to a few splitters; for keys
-// crowded under one prefix it is the full list and the search is the plain
-// binary search.  Built by every workgroup from the splitters in LDS.
+// Any prefix that does not decrease as the key grows keeps this exact.  For
+// spread-out keys the range holds zero to a few splitters; for keys crowded
+// under one prefix it is the full list and the search is the plain binary
+// search.  Built by every workgroup from the splitters in LDS.
 constexpr int kPartLutBits = 12;
 constexpr int kPartLutSize = (1 << kPartLutBits) + 1;
 
-template <class K> __device__ __forceinline__ uint32_t part_prefix(K k) {
+template <class K> __device__ __forceinline__ uint64_t part_image(K k) {
     if constexpr (sizeof(K) == 4) {
-        const uint32_t u = std::is_signed<K>::value ? uint32_t(k) ^ 0x80000000u : uint32_t(k);
-        return u >> (32 - kPartLutBits);
+        return std::is_signed<K>::value ? uint64_t(uint32_t(k) ^ 0x80000000u) : uint64_t(uint32_t(k));
     } else {
-        const uint64_t u = std::is_signed<K>::value ? uint64_t(k) ^ (uint64_t(1) << 63) : uint64_t(k);
-        return uint32_t(u >> (64 - kPartLutBits));
+        return std::is_signed<K>::value ? uint64_t(k) ^ (uint64_t(1) << 63) : uint64_t(k);
     }
 }
 
+struct PartLutMap {   // prefix = min((image - base) >> shift, 2^kPartLutBits - 1), 0 below base
+    uint64_t base;
+    int shift;
+};
+
+template <class K> __device__ __forceinline__ uint32_t part_prefix(K k, PartLutMap m) {
+    if constexpr (sizeof(K) == 4) {   // 32-bit arithmetic: base and shift fit
+        const uint32_t u = uint32_t(part_image(k)), b = uint32_t(m.base);
+        const uint32_t r = u > b ? (u - b) >> m.shift : 0u;
+        return r < uint32_t(kPartLutSize - 1) ? r : uint32_t(kPartLutSize - 2);
+    } else {
+        const uint64_t u = part_image(k);
+        const uint64_t r = u > m.base ? (u - m.base) >> m.shift : 0;
+        return r < uint64_t(kPartLutSize - 1) ? uint32_t(r) : uint32_t(kPartLutSize - 2);
+    }
+}
+
+// Map of a segment whose keys lie in [image(sp[lo]), image(sp[hi])] (whole
+// key range where unbounded).
+template <class K> __device__ __forceinline__ PartLutMap part_lut_map(const K* sp_all, int lo, int hi) {
+    const uint64_t top = sizeof(K) == 4 ? 0xFFFFFFFFull : ~uint64_t(0);
+    const uint64_t base = lo >= 0 ? part_image(sp_all[lo]) : 0;
+    const uint64_t end = hi >= 0 ? part_image(sp_all[hi]) : top;
+    const uint64_t range = end > base ? end - base : 0;
+    const int bits = range ? 64 - __clzll(range) : 0;
+    return PartLutMap{base, bits > kPartLutBits ? bits - kPartLutBits : 0};
+}
+
 template <class K>
-__device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint16_t* lut) {
+__device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint16_t* lut, PartLutMap m) {
     for (int x = int(threadIdx.x); x < kPartLutSize; x += int(blockDim.x)) {
         int lo = 0, hi = ns;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (part_prefix(sp[mid]) < uint32_t(x)) lo = mid + 1;
+            if (part_prefix(sp[mid], m) < uint32_t(x)) lo = mid + 1;
             else hi = mid;
         }
         lut[x] = uint16_t(lo);
@@ -96,114 +160,147 @@ __device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint16_t* lu
 // Bucket of key k: j = number of splitters <= k; a key equal to splitter
 // j-1 goes to the equality bucket 2j-1, any other key to bucket 2j.
 template <class E>
-__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, const uint16_t* lut) {
-    const uint32_t x = part_prefix(k);
+__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, const uint16_t* lut,
+                                           PartLutMap m) {
+    const uint32_t x = part_prefix(k, m);
     int lo = lut[x], hi = lut[x + 1];
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (!(k < sp[mid])) lo = mid + 1;
         else hi = mid;
     }
-    return (lo > 0 && !(sp[lo - 1] < k)) ? 2 * lo - 1 : 2 * lo;
+    const typename E::Key below = sp[lo > 0 ? lo - 1 : 0];   // read unconditionally: no branch
+    return (lo > 0 && !(below < k)) ? 2 * lo - 1 : 2 * lo;
 }
 
-// Per-block histogram: every thread classifies part_items keys per tile,
+// Per-workgroup histogram: every thread classifies part_items keys per tile,
 // their loads all in flight together (coalesced across the workgroup).
 template <int DT>
 __global__ void __launch_bounds__(kPartThreads)
-k_part_count(const typename Elem<DT>::Storage* src, int64_t n, const typename Elem<DT>::Key* splitters, int ns,
-             int64_t per_block, int blocks, int64_t* counts) {
+k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
+             int64_t* counts) {
     using E = Elem<DT>;
     constexpr int IT = part_items<DT>();
     __shared__ typename E::Key sp[kPartMaxSplitters];
     __shared__ int hist[kPartMaxBuckets];
     __shared__ uint16_t lut[kPartLutSize];
+    const PartBlock w = work[blockIdx.x];
+    const int ns = w.ns;
     const int nb = 2 * ns + 1;
     const int tid = int(threadIdx.x);
-    for (int i = tid; i < ns; i += kPartThreads) sp[i] = splitters[i];
+    for (int i = tid; i < ns; i += kPartThreads) sp[i] = splitters[w.sp0 + i];
     for (int i = tid; i < nb; i += kPartThreads) hist[i] = 0;
+    const PartLutMap map = part_lut_map(splitters, w.lo, w.hi);
     __syncthreads();
-    part_build_lut(sp, ns, lut);
+    part_build_lut(sp, ns, lut, map);
     __syncthreads();
-    const int64_t b0 = int64_t(blockIdx.x) * per_block;
-    const int64_t b1 = b0 + per_block < n ? b0 + per_block : n;
-    for (int64_t t0 = b0; t0 < b1; t0 += int64_t(IT) * kPartThreads) {
-        typename E::Key key[IT];
+    const int64_t b1 = w.end;
+    constexpr int64_t STEP = int64_t(IT) * kPartThreads;
+    // the next tile's loads are issued before this tile is classified
+    // Loads past the end re-read the last element (unconditional loads keep
+    // the prefetched tile in flight; their results are masked below).
+    const int64_t last = b1 - 1;
+    typename E::Storage cur[IT];
+    #pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int64_t i = w.begin + int64_t(k) * kPartThreads + tid;
+        cur[k] = src[i < b1 ? i : last];
+    }
+    for (int64_t t0 = w.begin; t0 < b1; t0 += STEP) {
+        typename E::Storage nxt[IT];
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const int64_t i = t0 + int64_t(k) * kPartThreads + tid;
-            key[k] = i < b1 ? E::key(src[i]) : typename E::Key{};
+            const int64_t i = t0 + STEP + int64_t(k) * kPartThreads + tid;
+            nxt[k] = src[i < b1 ? i : last];
         }
         #pragma unroll
         for (int k = 0; k < IT; ++k)
-            if (t0 + int64_t(k) * kPartThreads + tid < b1) atomicAdd(&hist[part_bucket<E>(key[k], sp, lut)], 1);
+            if (t0 + int64_t(k) * kPartThreads + tid < b1)
+                atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) cur[k] = nxt[k];
     }
     __syncthreads();
-    for (int b = tid; b < nb; b += kPartThreads) counts[int64_t(b) * blocks + blockIdx.x] = hist[b];
+    for (int b = tid; b < nb; b += kPartThreads) counts[w.cell + int64_t(b) * w.stride] = hist[b];
 }
 
-// Dynamic LDS of k_part_scatter for nb buckets: splitters, tile histogram /
-// offsets, the block's running output positions, the tile staged in bucket
-// order and the bucket of every staged element.
+// Dynamic LDS of k_part_scatter for up to ns splitters per workgroup:
+// splitters, tile histogram / offsets, the workgroup's running output
+// positions, the tile staged in bucket order and the bucket of every staged
+// element.
 constexpr size_t part_align(size_t x) { return (x + 15) & ~size_t(15); }
 template <int DT> size_t part_scatter_lds(int ns) {
     using E = Elem<DT>;
     const size_t nb = size_t(2 * ns + 1);
     return part_align(size_t(ns > 0 ? ns : 1) * sizeof(typename E::Key)) + part_align(nb * 4) +
-           part_align(nb * 8) + part_align(size_t(part_tile<DT>()) * sizeof(typename E::Storage)) +
-           part_align(size_t(part_tile<DT>()) * 2) + 64;
+           part_align(nb * 8) + part_align(size_t(part_scatter_tile<DT>()) * sizeof(typename E::Storage)) +
+           part_align(size_t(part_scatter_tile<DT>()) * 2) + 64;
 }
 
 // Scatter.  Per tile of part_tile elements: classify, rank inside the tile
 // with LDS atomics, scan the tile histogram, stage the tile in bucket order
 // in LDS, then write it out with consecutive threads on consecutive
 // positions of each bucket's run (coalesced where the reference's
-// per-element atomics scatter every key to its own address).
+// per-element atomics scatter every key to its own address).  max_ns sizes
+// the LDS carve-up (the largest ns of any workgroup of the launch).
 template <int DT>
-__global__ void __launch_bounds__(kPartThreads)
-k_part_scatter(const typename Elem<DT>::Storage* src, int64_t n, const typename Elem<DT>::Key* splitters, int ns,
-               int64_t per_block, int blocks, const int64_t* offsets, typename Elem<DT>::Storage* dst) {
+__global__ void __launch_bounds__(part_scatter_threads<DT>())
+k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
+               int max_ns, const int64_t* offsets, typename Elem<DT>::Storage* dst) {
     using E = Elem<DT>;
     using K = typename E::Key;
     using S = typename E::Storage;
     constexpr int IT = part_items<DT>();
-    constexpr int TILE = part_tile<DT>();
+    constexpr int NT = part_scatter_threads<DT>();
+    constexpr int TILE = part_scatter_tile<DT>();
     extern __shared__ __align__(16) unsigned char part_lds[];
+    const PartBlock w = work[blockIdx.x];
+    const int ns = w.ns;
     const int nb = 2 * ns + 1;
+    const int max_nb = 2 * max_ns + 1;
     K* sp = reinterpret_cast<K*>(part_lds);
-    int* toff = reinterpret_cast<int*>(part_lds + part_align(size_t(ns > 0 ? ns : 1) * sizeof(K)));
-    int64_t* run = reinterpret_cast<int64_t*>(reinterpret_cast<unsigned char*>(toff) + part_align(size_t(nb) * 4));
-    S* stage = reinterpret_cast<S*>(reinterpret_cast<unsigned char*>(run) + part_align(size_t(nb) * 8));
+    int* toff = reinterpret_cast<int*>(part_lds + part_align(size_t(max_ns > 0 ? max_ns : 1) * sizeof(K)));
+    int64_t* run = reinterpret_cast<int64_t*>(reinterpret_cast<unsigned char*>(toff) + part_align(size_t(max_nb) * 4));
+    S* stage = reinterpret_cast<S*>(reinterpret_cast<unsigned char*>(run) + part_align(size_t(max_nb) * 8));
     uint16_t* sbkt = reinterpret_cast<uint16_t*>(reinterpret_cast<unsigned char*>(stage) +
                                                  part_align(size_t(TILE) * sizeof(S)));
-    __shared__ int wsum[kPartThreads / 64];
+    __shared__ int wsum[NT / 64];
     __shared__ uint16_t lut[kPartLutSize];
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < ns; i += kPartThreads) sp[i] = splitters[i];
-    for (int b = tid; b < nb; b += kPartThreads) {
+    for (int i = tid; i < ns; i += NT) sp[i] = splitters[w.sp0 + i];
+    for (int b = tid; b < nb; b += NT) {
         toff[b] = 0;
-        run[b] = offsets[int64_t(b) * blocks + blockIdx.x];
+        run[b] = offsets[w.cell + int64_t(b) * w.stride];
     }
+    const PartLutMap map = part_lut_map(splitters, w.lo, w.hi);
     __syncthreads();
-    part_build_lut(sp, ns, lut);
+    part_build_lut(sp, ns, lut, map);
     __syncthreads();
-    const int per_thread = (nb + kPartThreads - 1) / kPartThreads;   // scan chunk of each thread
-    const int64_t b0 = int64_t(blockIdx.x) * per_block;
-    const int64_t b1 = b0 + per_block < n ? b0 + per_block : n;
-    for (int64_t t0 = b0; t0 < b1; t0 += TILE) {
+    const int per_thread = (nb + NT - 1) / NT;   // scan chunk of each thread
+    const int64_t b1 = w.end;
+    // the next tile's loads are issued before this tile is processed
+    // (loads past the end re-read the last element, as in the count kernel)
+    const int64_t last = b1 - 1;
+    S elem[IT];
+    #pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int64_t i = w.begin + int64_t(k) * NT + tid;
+        elem[k] = src[i < b1 ? i : last];
+    }
+    for (int64_t t0 = w.begin; t0 < b1; t0 += TILE) {
         const int tn = int(b1 - t0 < TILE ? b1 - t0 : TILE);
-        S elem[IT];
+        S nxt[IT];
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const int64_t i = t0 + TILE + int64_t(k) * NT + tid;
+            nxt[k] = src[i < b1 ? i : last];
+        }
         int bkt[IT], rank[IT];
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const int x = k * kPartThreads + tid;
-            elem[k] = x < tn ? src[t0 + x] : S{};
-        }
-        #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-            const int x = k * kPartThreads + tid;
-            bkt[k] = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut) : 0;
+            const int x = k * NT + tid;
+            bkt[k] = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
             rank[k] = x < tn ? atomicAdd(&toff[bkt[k]], 1) : 0;
         }
         __syncthreads();
@@ -221,7 +318,7 @@ k_part_scatter(const typename Elem<DT>::Storage* src, int64_t n, const typename 
         if (lane == 63) wsum[wave] = incl;
         __syncthreads();
         int acc = incl - csum;
-        for (int w = 0; w < wave; ++w) acc += wsum[w];
+        for (int q = 0; q < wave; ++q) acc += wsum[q];
         for (int b = c0; b < c0 + per_thread && b < nb; ++b) {
             const int c = toff[b];
             toff[b] = acc;
@@ -230,23 +327,36 @@ k_part_scatter(const typename Elem<DT>::Storage* src, int64_t n, const typename 
         __syncthreads();
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            if (k * kPartThreads + tid < tn) {
+            if (k * NT + tid < tn) {
                 const int pos = toff[bkt[k]] + rank[k];
                 stage[pos] = elem[k];
                 sbkt[pos] = uint16_t(bkt[k]);
             }
         }
         __syncthreads();
-        for (int j = tid; j < tn; j += kPartThreads) {
+        for (int j = tid; j < tn; j += NT) {
             const int b = sbkt[j];
             dst[run[b] + (j - toff[b])] = stage[j];
         }
         __syncthreads();
-        for (int b = tid; b < nb; b += kPartThreads) run[b] += (b + 1 < nb ? toff[b + 1] : tn) - toff[b];
+        for (int b = tid; b < nb; b += NT) run[b] += (b + 1 < nb ? toff[b + 1] : tn) - toff[b];
         __syncthreads();   // every toff[b + 1] read before any is cleared
-        for (int b = tid; b < nb; b += kPartThreads) toff[b] = 0;
+        for (int b = tid; b < nb; b += NT) toff[b] = 0;
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) elem[k] = nxt[k];
         __syncthreads();
     }
+}
+
+// Bucket starts of every segment (one workgroup per segment).
+__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts);
+
+// The second level's splitters of first-level bucket j are every parts2-th
+// global splitter: first-level splitter j = global splitter (j+1)*parts2 - 1.
+template <class K>
+__global__ void k_part_gather(const K* global, int parts1, int parts2, K* level1) {
+    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j < parts1 - 1) level1[j] = global[int64_t(j + 1) * parts2 - 1];
 }
 
 // Exclusive scan of int64 values in place, three launches: per-block sums,
@@ -258,7 +368,6 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 __global__ void k_scan_sums(const int64_t* v, int64_t n, int64_t* sums);
 __global__ void k_scan_top(int64_t* sums, int64_t nblocks);
 __global__ void k_scan_apply(int64_t* v, int64_t n, const int64_t* sums);
-__global__ void k_bucket_totals(const int64_t* offsets, int nb, int blocks, int64_t n, int64_t* bucket_off);
 
 // Merge pass over a list of oversized buckets: item = (bucket base, bucket
 // length, first output position of this tile inside the bucket), plus the
@@ -306,10 +415,11 @@ k_seg_merge_pass(const typename Elem<DT>::Storage* src, typename Elem<DT>::Stora
 struct PartKernels {
     hipError_t (*sample)(const void* src, int64_t n, void* samples, int m, hipStream_t s) = nullptr;
     hipError_t (*splitters)(const void* sorted_samples, int m, int parts, void* splitters, hipStream_t s) = nullptr;
-    hipError_t (*count)(const void* src, int64_t n, const void* splitters, int ns, int64_t per_block, int blocks,
-                        int64_t* counts, hipStream_t s) = nullptr;
-    hipError_t (*scatter)(const void* src, int64_t n, const void* splitters, int ns, int64_t per_block, int blocks,
+    hipError_t (*count)(const void* src, const void* splitters, const PartBlock* work, int blocks, int64_t* counts,
+                        hipStream_t s) = nullptr;
+    hipError_t (*scatter)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
                           const int64_t* offsets, void* dst, hipStream_t s) = nullptr;
+    hipError_t (*gather)(const void* global, int parts1, int parts2, void* level1, hipStream_t s) = nullptr;
     // Co-rank search + merge pass (or a plain copy with copy_only); writes
     // the co-ranks into items.
     hipError_t (*seg_merge)(const void* src, void* dst, MergeTile* items, int nitems, int64_t run,
@@ -334,23 +444,29 @@ PartKernels part_kernels() {
                            static_cast<const S*>(sorted), m, parts, static_cast<K*>(sp));
         return hipGetLastError();
     };
-    k.count = [](const void* src, int64_t n, const void* sp, int ns, int64_t per_block, int blocks, int64_t* counts,
+    k.count = [](const void* src, const void* sp, const PartBlock* work, int blocks, int64_t* counts,
                  hipStream_t s) {
         hipLaunchKernelGGL(k_part_count<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), 0, s,
-                           static_cast<const S*>(src), n, static_cast<const K*>(sp), ns, per_block, blocks, counts);
+                           static_cast<const S*>(src), static_cast<const K*>(sp), work, counts);
         return hipGetLastError();
     };
-    k.scatter = [](const void* src, int64_t n, const void* sp, int ns, int64_t per_block, int blocks,
+    k.scatter = [](const void* src, const void* sp, const PartBlock* work, int blocks, int max_ns,
                    const int64_t* offsets, void* dst, hipStream_t s) {
-        const size_t lds = part_scatter_lds<DT>(ns);
+        const size_t lds = part_scatter_lds<DT>(max_ns);
         if (lds > 64 * 1024) {
             const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_scatter<DT>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(k_part_scatter<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), lds, s,
-                           static_cast<const S*>(src), n, static_cast<const K*>(sp), ns, per_block, blocks, offsets,
+        hipLaunchKernelGGL(k_part_scatter<DT>, dim3(unsigned(blocks)), dim3(part_scatter_threads<DT>()), lds, s,
+                           static_cast<const S*>(src), static_cast<const K*>(sp), work, max_ns, offsets,
                            static_cast<S*>(dst));
+        return hipGetLastError();
+    };
+    k.gather = [](const void* global, int parts1, int parts2, void* level1, hipStream_t s) {
+        if (parts1 < 2) return hipSuccess;
+        hipLaunchKernelGGL(k_part_gather<K>, dim3(unsigned((parts1 + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const K*>(global), parts1, parts2, static_cast<K*>(level1));
         return hipGetLastError();
     };
     k.seg_merge = [](const void* src, void* dst, MergeTile* items, int nitems, int64_t run, int copy_only,

@@ -66,10 +66,8 @@ void set_tuned_variant(int dtype, int b, int v) {
     g_variant[dtype][b] = v;
 }
 
-int bucket_of(int64_t len) {
-    int b = 0;
-    while ((int64_t(1) << b) < len) ++b;
-    return b;
+int bucket_of(int64_t len) {   // ceil(log2(len)), 0 for len <= 1
+    return len > 1 ? 64 - __builtin_clzll(uint64_t(len - 1)) : 0;
 }
 
 }  // namespace
@@ -247,37 +245,50 @@ int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* fo
                     std::vector<SegItem>& items, int64_t* max_len, int* variants, int64_t* bucket_segments) {
     const SegKernels k = seg_kernels_for(dtype);
     const int maxb = k.max_logw;
-    std::vector<std::vector<int>> per(kMaxBuckets);
+    // counting sort of the segments by bucket, largest bucket first
+    thread_local std::vector<unsigned char> bkt;   // kept mapped between calls
+    bkt.resize(static_cast<size_t>(segs));
+    int64_t count[kMaxBuckets] = {};
     int64_t mx = 0;
     for (int64_t s = 0; s < segs; ++s) {
         const int64_t len = h_off[s + 1] - h_off[s];
         if (len < 0) return CS_E_INVALID_VALUE;
         if (len > (int64_t(1) << maxb)) return CS_E_CAPACITY;
         mx = std::max(mx, len);
-        per[size_t(bucket_of(len))].push_back(int(s));
+        const int b = bucket_of(len);
+        bkt[size_t(s)] = static_cast<unsigned char>(b);
+        ++count[b];
     }
     if (max_len) *max_len = mx;
-    list.clear();
+    int64_t start[kMaxBuckets] = {};
+    for (int b = maxb, at = 0; b >= 0; --b) {
+        start[b] = at;
+        at += int(count[b]);
+    }
+    list.assign(size_t(segs), 0);
+    {
+        int64_t fill[kMaxBuckets];
+        std::copy(start, start + kMaxBuckets, fill);
+        for (int64_t s = 0; s < segs; ++s) list[size_t(fill[bkt[size_t(s)]]++)] = int(s);
+    }
     items.clear();
-    list.reserve(size_t(segs));
+    items.reserve(static_cast<size_t>(segs));
     for (int b = maxb; b >= 0; --b) {
-        const auto& seg = per[size_t(b)];
-        if (bucket_segments) bucket_segments[b] = int64_t(seg.size());
+        if (bucket_segments) bucket_segments[b] = count[b];
         int v = force_variant ? force_variant[b] : tuned_variant(dtype, b);
         if (v < 0 || !k.variant_ok(b, v)) v = 0;
         if (!k.variant_ok(b, v)) v = 2;
         if (variants) variants[b] = v;
-        if (seg.empty()) continue;
-        const int spi = k.segments_per_item(b, v);
-        for (size_t i = 0; i < seg.size(); i += size_t(spi)) {
+        if (count[b] == 0) continue;
+        const int64_t spi = k.segments_per_item(b, v);
+        for (int64_t i = 0; i < count[b]; i += spi) {
             SegItem it;
             it.bucket = b;
             it.variant = v;
-            it.first = int(list.size() + i);
-            it.count = int(std::min(seg.size() - i, size_t(spi)));
+            it.first = int(start[b] + i);
+            it.count = int(std::min(count[b] - i, spi));
             items.push_back(it);
         }
-        list.insert(list.end(), seg.begin(), seg.end());
     }
     return CS_OK;
 }
