@@ -239,14 +239,15 @@ int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream
 
 int64_t cs_hybrid_threshold(int dtype) {
     // Crossover measured on MI355X (uniform keys; merge / partition ms):
-    //   int32    2^24 0.59 / 0.95   2^26 2.59 / 2.41   2^28 10.4 / 8.6
-    //   uint64   2^24 1.25 / 1.56   2^26 5.44 / 4.83   2^27 10.9 / 10.3
-    //   int32+int32 records  2^24 0.98 / 1.20   2^26 4.24 / 3.64
-    //   int64+int32 records  2^22 1.11 / 1.28   2^24 5.55 / 3.66
+    //   int32                2^24 0.56 / 0.83   2^26 2.54 / 1.70   2^28 10.3 / 5.23
+    //   uint64               2^22 0.39 / 0.87   2^24 1.23 / 1.15   2^26 5.35 / 2.78
+    //   int32+int32 records  2^24 0.95 / 1.00   2^25 2.16 / 1.44
+    //   int64+int32 records  2^21 0.59 / 0.80   2^22 1.09 / 0.92
     // The reference encodes its own crossover as CuSortTuning
     // (cusort.hpp:2476-2483: never for 4-byte keys, 2^27 for 8-byte ones).
     if (!valid_dtype(dtype)) return 0;
-    return cs_dtype_size(dtype) >= 16 ? (int64_t(1) << 23) : (int64_t(1) << 25);
+    const size_t size = cs_dtype_size(dtype);
+    return int64_t(1) << (size >= 16 ? 22 : (size >= 8 ? 24 : 25));
 }
 
 }  // extern "C"

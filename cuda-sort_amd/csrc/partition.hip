@@ -86,6 +86,103 @@ __global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64
         starts[g.first + b] = offsets[g.cell0 + int64_t(b) * g.stride];
 }
 
+// ---------------------------------------------------------------- device plan
+// The final ragged sort is planned on the device from the last level's
+// bucket starts: every bucket is a segment, classed by ceil(log2 length);
+// the list holds the bucket ids grouped by class, largest class first, and
+// work items cover `spi` consecutive list entries of one class, as the host
+// planner lays them out (the order inside a class is free: every segment is
+// sorted on its own).  A bucket longer than the tile gets class kPlanLong
+// and sends the call to the host planner, which cuts it into chunks.
+constexpr int kPlanClasses = 16;
+constexpr int kPlanLong = 14;
+
+struct PlanTable {
+    int maxb;
+    int small_max;
+    int spi[kPlanClasses];
+    int variant[kPlanClasses];
+};
+
+// meta: ccount[16], cursor[16], cstart[16], istart[16], hdr[4]
+constexpr int kPlanMeta = 4 * kPlanClasses + 4;
+
+__global__ void k_plan_zero(int* meta) {
+    if (threadIdx.x < kPlanMeta) meta[threadIdx.x] = 0;
+}
+
+__global__ void k_plan_classify(int64_t* starts, int64_t nb, int64_t n, int64_t chunk, unsigned char* cls,
+                                int* ccount) {
+    __shared__ int h[kPlanClasses];
+    if (threadIdx.x < kPlanClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nb) {
+        const int64_t len = (i + 1 < nb ? starts[i + 1] : n) - starts[i];
+        int c = 255;
+        if (len > chunk) c = kPlanLong;
+        else if (len > 0) c = len > 1 ? 64 - __clzll(uint64_t(len - 1)) : 0;
+        cls[i] = static_cast<unsigned char>(c);
+        if (c < kPlanClasses) atomicAdd(&h[c], 1);
+    }
+    if (i == 0) starts[nb] = n;   // closing offset of the segment CSR
+    __syncthreads();
+    if (threadIdx.x < kPlanClasses && h[threadIdx.x]) atomicAdd(&ccount[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_plan_layout(int* meta, PlanTable t) {
+    if (threadIdx.x != 0) return;
+    const int* ccount = meta;
+    int* cstart = meta + 2 * kPlanClasses;
+    int* istart = meta + 3 * kPlanClasses;
+    int* hdr = meta + 4 * kPlanClasses;
+    int seg = 0, items = 0, split = 0;
+    for (int c = t.maxb; c >= 0; --c) {
+        const int k = (ccount[c] + t.spi[c] - 1) / t.spi[c];
+        cstart[c] = seg;
+        istart[c] = items;
+        seg += ccount[c];
+        items += k;
+        if (c > t.small_max) split += k;
+    }
+    hdr[0] = split;
+    hdr[1] = items;
+    hdr[2] = ccount[kPlanLong];
+}
+
+__global__ void k_plan_fill(const unsigned char* cls, int64_t nb, int* meta, int* list) {
+    __shared__ int h[kPlanClasses], base[kPlanClasses];
+    int* cursor = meta + kPlanClasses;
+    const int* cstart = meta + 2 * kPlanClasses;
+    if (threadIdx.x < kPlanClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int c = i < nb ? int(cls[i]) : 255;
+    const int local = c < kPlanLong ? atomicAdd(&h[c], 1) : 0;
+    __syncthreads();
+    if (threadIdx.x < kPlanClasses && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (c < kPlanLong) list[cstart[c] + base[c] + local] = int(i);
+}
+
+__global__ void k_plan_items(const int* meta, PlanTable t, SegItem* items) {
+    const int* ccount = meta;
+    const int* cstart = meta + 2 * kPlanClasses;
+    const int* istart = meta + 3 * kPlanClasses;
+    const int* hdr = meta + 4 * kPlanClasses;
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= hdr[1]) return;
+    int c = t.maxb;   // classes run from maxb down, their item ranges upward
+    while (c > 0 && k >= istart[c] + (ccount[c] + t.spi[c] - 1) / t.spi[c]) --c;
+    const int j = k - istart[c];
+    SegItem it;
+    it.bucket = c;
+    it.variant = t.variant[c];
+    it.first = cstart[c] + j * t.spi[c];
+    it.count = min(t.spi[c], ccount[c] - j * t.spi[c]);
+    items[k] = it;
+}
+
 PartKernels part_kernels_i32();
 PartKernels part_kernels_u32();
 PartKernels part_kernels_f32();
@@ -187,18 +284,18 @@ void add_segment(Level& L, int64_t begin, int64_t end, int sp0, int ns, int lo, 
     L.max_ns = std::max(L.max_ns, ns);
 }
 
-// Count, scan and scatter of one level (src -> dst), then the bucket starts
-// read back into `starts` (one host synchronisation).
+// Count, scan and scatter of one level (src -> dst) and its bucket starts in
+// d_starts (nbuckets + 1 entries); with h_starts they are also read back
+// (one host synchronisation).
 hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, const void* splitters, void* dst,
-                     std::vector<int64_t>& starts, hipStream_t stream) {
+                     int64_t* d_starts, std::vector<int64_t>* h_starts, hipStream_t stream) {
     Scoped<PartBlock> work(stream);
     Scoped<PartSeg> segs(stream);
-    Scoped<int64_t> counts(stream), sums(stream), d_starts(stream);
+    Scoped<int64_t> counts(stream), sums(stream);
     hipError_t e = work.alloc(L.work.size());
     if (e == hipSuccess) e = segs.alloc(L.segs.size());
     if (e == hipSuccess) e = counts.alloc(size_t(L.cells));
     if (e == hipSuccess) e = sums.alloc(size_t((L.cells + kScanTile - 1) / kScanTile) + 1);
-    if (e == hipSuccess) e = d_starts.alloc(size_t(L.nbuckets));
     if (e == hipSuccess)
         e = hipMemcpyAsync(work.p, L.work.data(), L.work.size() * sizeof(PartBlock), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess)
@@ -209,13 +306,16 @@ hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, con
     if (e == hipSuccess) e = pk.scatter(src, splitters, work.p, blocks, L.max_ns, counts.p, dst, stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_part_starts, dim3(unsigned(L.segs.size())), dim3(256), 0, stream, segs.p, counts.p,
-                           d_starts.p);
+                           d_starts);
         e = hipGetLastError();
     }
-    starts.assign(size_t(L.nbuckets), 0);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(starts.data(), d_starts.p, starts.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (h_starts) {
+        h_starts->assign(size_t(L.nbuckets), 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_starts->data(), d_starts, h_starts->size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
     return e;
 }
 
@@ -282,7 +382,10 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
     clear_level(L2);
     add_segment(L1, 0, n, 0, p1 - 1, -1, -1, false);
     std::vector<int64_t>& st1 = host.st1;
-    e = run_level(pk, L1, data, p2 > 1 ? l1sp.p : gsp.p, tmp, st1, stream);
+    Scoped<int64_t> d_st1(stream), d_st2(stream);
+    e = d_st1.alloc(size_t(L1.nbuckets) + 1);
+    if (e == hipSuccess)
+        e = run_level(pk, L1, data, p2 > 1 ? l1sp.p : gsp.p, tmp, d_st1.p, p2 > 1 ? &st1 : nullptr, stream);
     if (e != hipSuccess) return int(e);
 
     // level 2: tmp -> out, one segment per first-level bucket; equality
@@ -303,12 +406,64 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
                             j < p1 - 1 ? (j + 1) * p2 - 1 : -1, false);
             }
         }
-        e = run_level(pk, L2, tmp, gsp.p, out, st2, stream);
+        e = d_st2.alloc(size_t(L2.nbuckets) + 1);
+        if (e == hipSuccess) e = run_level(pk, L2, tmp, gsp.p, out, d_st2.p, nullptr, stream);
         if (e != hipSuccess) return int(e);
         fin = &L2;
         fst = &st2;
         src = out;
     }
+    int64_t* d_fst = p2 > 1 ? d_st2.p : d_st1.p;
+    const int64_t fnb = fin->nbuckets;
+
+    // plan the final ragged sort on the device; only its header comes back
+    {
+        const SegClasses sc = seg_classes(dtype);
+        PlanTable t{};
+        t.maxb = sc.maxb;
+        t.small_max = sc.small_max;
+        for (int c = 0; c < kPlanClasses; ++c) {
+            t.spi[c] = c <= sc.maxb ? std::max(1, sc.spi[c]) : 1;
+            t.variant[c] = c <= sc.maxb ? sc.variant[c] : 0;
+        }
+        Scoped<unsigned char> cls(stream);
+        Scoped<int> meta(stream), d_list(stream), d_counters(stream);
+        Scoped<SegItem> d_items(stream);
+        e = cls.alloc(size_t(fnb));
+        if (e == hipSuccess) e = meta.alloc(kPlanMeta);
+        if (e == hipSuccess) e = d_list.alloc(size_t(fnb));
+        if (e == hipSuccess) e = d_items.alloc(size_t(fnb));
+        if (e == hipSuccess) e = d_counters.alloc(2);
+        if (e != hipSuccess) return int(e);
+        const unsigned g = unsigned((fnb + 255) / 256);
+        hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(128), 0, stream, meta.p);
+        hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, fnb, n, lk.chunk, cls.p, meta.p);
+        hipLaunchKernelGGL(k_plan_layout, dim3(1), dim3(64), 0, stream, meta.p, t);
+        hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls.p, fnb, meta.p, d_list.p);
+        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta.p, t, d_items.p);
+        int hdr[3] = {0, 0, 0};
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hdr, meta.p + 4 * kPlanClasses, sizeof(hdr), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return int(e);
+        if (hdr[2] == 0) {   // no bucket longer than the tile: sort with the device plan
+            SegShape sh;
+            sh.grid[kSegMulti] = seg_persistent_grid(dtype, kSegMulti, 1 << 30);
+            sh.grid[kSegSingle] = seg_persistent_grid(dtype, kSegSingle, 1 << 30);
+            if (sh.grid[kSegMulti] <= 0 || sh.grid[kSegSingle] <= 0) return CS_E_NO_DEVICE;
+            e = seg_launch_all(dtype, sh, src, out, d_fst, d_list.p, d_items.p, 0, d_counters.p, stream,
+                               meta.p + 4 * kPlanClasses);
+            return e == hipSuccess ? CS_OK : int(e);
+        }
+    }
+    // some bucket is longer than the tile: the host plans the sort and
+    // cuts the long buckets into chunks
+    std::vector<int64_t>& fst_host = p2 > 1 ? st2 : st1;
+    fst_host.assign(size_t(fnb), 0);
+    e = hipMemcpyAsync(fst_host.data(), d_fst, size_t(fnb) * sizeof(int64_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return int(e);
 
     // one ragged batch over all buckets (src -> out); long buckets cut into
     // chunks.  The level's bucket starts run through the array in order.

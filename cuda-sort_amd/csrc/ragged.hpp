@@ -44,9 +44,22 @@ hipError_t ragged_launch(int dtype, const void* in, void* out, void* scratch, co
 int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* force_variant, std::vector<int>& list,
                     std::vector<SegItem>& items, int64_t* max_len, int* variants, int64_t* bucket_segments);
 int seg_persistent_grid(int dtype, int which, int nitems);
+// Per length class (bucket) b <= maxb: the variant seg_build_items picks and
+// its segments per work item.
+struct SegClasses {
+    int maxb = 0;
+    int small_max = 0;
+    int variant[16] = {};
+    int spi[16] = {};
+};
+SegClasses seg_classes(int dtype);
 SegShape seg_shape(int dtype, const std::vector<SegItem>& items);
-// Counter reset and both persistent launches, stream ordered.
+// Counter reset and both persistent launches, stream ordered.  With d_range
+// (two ints in device memory: items of the multi-wave kernel, all items)
+// the item ranges come from a plan built on the device and `sh` only gives
+// the grid sizes.
 hipError_t seg_launch_all(int dtype, const SegShape& sh, const void* in, void* out, const int64_t* off,
-                          const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s);
+                          const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s,
+                          const int* d_range = nullptr);
 
 }  // namespace cs

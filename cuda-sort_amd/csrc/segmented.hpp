@@ -229,12 +229,19 @@ template <int DT, int BLO, int BHI> constexpr int seg_waves_per_simd() {
 }
 
 // Persistent kernel for the items [begin, end) of the list, whose buckets
-// all lie in BLO..BHI; *counter starts at 0.
+// all lie in BLO..BHI; *counter starts at 0.  With `range` (a plan built on
+// the device) the item range is read from it instead: range[0] items of the
+// multi-wave kernel (BLO > 0) come first, then the single-wave kernel's up
+// to range[1].
 template <int DT, int BLO, int BHI>
 __global__ void __launch_bounds__(kSegBlock, (seg_waves_per_simd<DT, BLO, BHI>()))
 k_seg_persistent(const typename Elem<DT>::Storage* in, typename Elem<DT>::Storage* out, const int64_t* off,
-                 const int* list, const SegItem* items, int begin, int end, int* counter) {
+                 const int* list, const SegItem* items, int begin, int end, int* counter, const int* range) {
     using E = Elem<DT>;
+    if (range) {
+        begin = BLO > 0 ? 0 : range[0];
+        end = BLO > 0 ? range[0] : range[1];
+    }
     __shared__ typename E::Key lds_k[seg_lds_words<DT, BLO, BHI>()];
     __shared__ int32_t lds_p[E::kPayload ? seg_lds_words<DT, BLO, BHI>() : 1];
     __shared__ int item_slot;
@@ -282,7 +289,7 @@ constexpr int kSegSingle = 1;
 // Host side (segmented_<dtype>.hip).
 struct SegKernels {
     hipError_t (*persistent)(int which, const void* in, void* out, const int64_t* off, const int* list,
-                             const SegItem* items, int begin, int end, int* counter, int grid,
+                             const SegItem* items, int begin, int end, int* counter, const int* range, int grid,
                              hipStream_t s) = nullptr;
     const void* persistent_fn[2] = {nullptr, nullptr};
     int small_max = 0;   // largest bucket of the kSegSingle kernel
@@ -295,17 +302,19 @@ struct SegKernels {
 
 template <int DT>
 hipError_t seg_launch_persistent(int which, const void* in, void* out, const int64_t* off, const int* list,
-                                 const SegItem* items, int begin, int end, int* counter, int grid, hipStream_t s) {
+                                 const SegItem* items, int begin, int end, int* counter, const int* range, int grid,
+                                 hipStream_t s) {
     using S = typename Elem<DT>::Storage;
     constexpr int SM = seg_small_max<DT>();
-    if (end <= begin || grid <= 0) return hipSuccess;
+    if ((!range && end <= begin) || grid <= 0) return hipSuccess;
     if (which == kSegMulti)
         hipLaunchKernelGGL((k_seg_persistent<DT, SM + 1, seg_max_logw<DT>()>), dim3(unsigned(grid)), dim3(kSegBlock),
                            0, s, static_cast<const S*>(in), static_cast<S*>(out), off, list, items, begin, end,
-                           counter);
+                           counter, range);
     else
         hipLaunchKernelGGL((k_seg_persistent<DT, 0, SM>), dim3(unsigned(grid)), dim3(kSegBlock), 0, s,
-                           static_cast<const S*>(in), static_cast<S*>(out), off, list, items, begin, end, counter);
+                           static_cast<const S*>(in), static_cast<S*>(out), off, list, items, begin, end, counter,
+                           range);
     return hipGetLastError();
 }
 

@@ -97,7 +97,7 @@ struct cs_segplan {
     const void* g_in = nullptr;
     void* g_out = nullptr;
     const int64_t* g_off = nullptr;
-    void* args[2][8] = {};
+    void* args[2][9] = {};
     void* reset_args[1] = {};
     const void* a_in = nullptr;
     void* a_out = nullptr;
@@ -108,6 +108,7 @@ struct cs_segplan {
     int a_end[2] = {0, 0};
     int* a_counter[2] = {nullptr, nullptr};
     int* a_reset = nullptr;
+    const int* a_range = nullptr;   // item ranges come from the host
 };
 
 namespace {
@@ -160,7 +161,7 @@ hipError_t build_graph(cs_segplan* p, const void* in, void* out, const int64_t* 
         p->a_counter[w] = p->d_counter + w;
         void** a = p->args[w];
         a[0] = &p->a_in; a[1] = &p->a_out; a[2] = &p->a_off; a[3] = &p->a_list; a[4] = &p->a_items;
-        a[5] = &p->a_begin[w]; a[6] = &p->a_end[w]; a[7] = &p->a_counter[w];
+        a[5] = &p->a_begin[w]; a[6] = &p->a_end[w]; a[7] = &p->a_counter[w]; a[8] = &p->a_range;
         if (p->a_end[w] <= p->a_begin[w]) continue;
         const hipKernelNodeParams kp = sort_node_params(p, w);
         e = hipGraphAddKernelNode(&p->sort_node[w], p->graph, &p->reset_node, 1, &kp);
@@ -240,6 +241,21 @@ int run_plan(cs_segplan* p, const void* in, void* out, const int64_t* d_offsets,
 }  // namespace
 
 namespace cs {
+
+SegClasses seg_classes(int dtype) {
+    const SegKernels k = seg_kernels_for(dtype);
+    SegClasses c;
+    c.maxb = k.max_logw;
+    c.small_max = k.small_max;
+    for (int b = 0; b <= c.maxb; ++b) {
+        int v = tuned_variant(dtype, b);
+        if (v < 0 || !k.variant_ok(b, v)) v = 0;
+        if (!k.variant_ok(b, v)) v = 2;
+        c.variant[b] = v;
+        c.spi[b] = k.segments_per_item(b, v);
+    }
+    return c;
+}
 
 int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* force_variant, std::vector<int>& list,
                     std::vector<SegItem>& items, int64_t* max_len, int* variants, int64_t* bucket_segments) {
@@ -322,15 +338,17 @@ SegShape seg_shape(int dtype, const std::vector<SegItem>& items) {
 }
 
 hipError_t seg_launch_all(int dtype, const SegShape& sh, const void* in, void* out, const int64_t* off,
-                          const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s) {
-    if (nitems <= 0) return hipSuccess;
+                          const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s,
+                          const int* d_range) {
+    if (!d_range && nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counters);
     hipError_t e = hipGetLastError();
     const SegKernels k = seg_kernels_for(dtype);
     if (e == hipSuccess)
-        e = k.persistent(kSegMulti, in, out, off, d_list, d_items, 0, sh.split, d_counters, sh.grid[kSegMulti], s);
+        e = k.persistent(kSegMulti, in, out, off, d_list, d_items, 0, sh.split, d_counters, d_range,
+                         sh.grid[kSegMulti], s);
     if (e == hipSuccess)
-        e = k.persistent(kSegSingle, in, out, off, d_list, d_items, sh.split, nitems, d_counters + 1,
+        e = k.persistent(kSegSingle, in, out, off, d_list, d_items, sh.split, nitems, d_counters + 1, d_range,
                          sh.grid[kSegSingle], s);
     return e;
 }
