@@ -38,8 +38,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_i32",
-                    choices=["batched_i32", "batched_rec32", "copy", "ragged_i32", "ragged_f32",
-                             "large_i32", "large_u64", "serve"])
+                    choices=["batched_i32", "batched_rec32", "batched_pairs64", "copy", "ragged_i32",
+                             "ragged_f32", "large_i32", "large_u64", "serve"])
     ap.add_argument("--rate", type=float, default=20000.0)
     ap.add_argument("--seconds", type=float, default=10.0)
     ap.add_argument("--budget-gb", type=float, default=32.0)
@@ -158,6 +158,71 @@ def bench_batched(args, dtype_name: str):
         "validated": ok,
     }
     return line
+
+
+def bench_pairs(args):
+    """Key / payload pairs in two arrays (int64 keys, int32 payloads), the
+    SoA form of the {int64 key, int32 payload} record: --arrays x --n pairs
+    sorted by cs_batched_sort_pairs, timed like the batched sort, next to the
+    padded 16-byte record sort of the same data (cs_batched_sort,
+    CS_REC_I64_I32)."""
+    import numpy as np
+    import cusort_hip as cs
+    import hiprt
+
+    n, arrays = args.n, args.arrays
+    count = n * arrays
+    hiprt.set_device(0)
+    stream = hiprt.Stream()
+    rec = cs.device_buffer(cs.REC_I64_I32, count)
+    rec_out = cs.device_buffer(cs.REC_I64_I32, count)
+    seed = 42
+    cs.generate(cs.REC_I64_I32, cs.D_UNIFORM, n, count, seed, rec.ptr, stream.handle)
+    stream.synchronize()
+    host = cs.to_numpy(rec, cs.REC_I64_I32, count)
+    keys = np.ascontiguousarray(host["key"])
+    vals = np.ascontiguousarray(host["payload"]).astype(np.int32)
+    kin, vin = cs.from_numpy(keys), cs.from_numpy(vals)
+    kout, vout = cs.from_numpy(np.zeros_like(keys)), cs.from_numpy(np.zeros_like(vals))
+
+    def timed(step):
+        warm_up(step, args.warmup)
+        start, end = hiprt.Event(), hiprt.Event()
+        start.record(stream)
+        for _ in range(args.steps):
+            step()
+        end.record(stream)
+        end.synchronize()
+        return start.elapsed_ms(end) / args.steps
+
+    ms = timed(lambda: cs.batched_sort_pairs(cs.I64, kin.ptr, vin.ptr, kout.ptr, vout.ptr, arrays, n,
+                                             stream.handle))
+    ms_rec = timed(lambda: cs.batched_sort(cs.REC_I64_I32, rec.ptr, rec_out.ptr, arrays, n, 0, 0, 0,
+                                           stream.handle))
+    got_k, got_v = kout.download(np.int64, count), vout.download(np.int32, count)
+    want = cs.to_numpy(rec_out, cs.REC_I64_I32, count)
+    same = got_k.tobytes() == np.ascontiguousarray(want["key"]).tobytes() and \
+        got_v.tobytes() == np.ascontiguousarray(want["payload"]).astype(np.int32).tobytes()
+    ok, cpu = None, None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        ok = same and oracle.validate_batched(cs.REC_I64_I32, host, want, n)[0] == 0
+        secs, threads = oracle.time_batched(cs.REC_I64_I32, host, n, threads=0, trials=args.cpu_trials)
+        cpu = {"value": count / secs / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
+               "sample": f"the full {arrays}x{n} batch as records, per-array std::sort over {threads} host "
+                         f"threads, median of {args.cpu_trials} trials"}
+    alg_bytes = count * 24   # 12 bytes per pair, read and written
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {"metric": METRIC, "value": count / (ms * 1e-3) / 1e9, "unit": "Gkeys/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int64+int32", "data": "synthetic",
+            "config": {"workload": f"batched pair sort, {arrays} arrays x {n} (int64 key, int32 payload) pairs in "
+                                   f"two arrays, uniform keys, seed {seed}", "arrays": arrays, "n": n},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+            "record_sort_ms": ms_rec, "same_as_record_sort": same,
+            "cpu_baseline": cpu, "validated": ok}
 
 
 def bench_copy(args):
@@ -360,6 +425,8 @@ def main():
         line = bench_batched(args, "i32")
     elif args.workload == "copy":
         line = bench_copy(args)
+    elif args.workload == "batched_pairs64":
+        line = bench_pairs(args)
     elif args.workload == "serve":
         line = bench_serve(args)
     elif args.workload.startswith("large"):

@@ -69,6 +69,8 @@ def lib():
         L.cs_method_from_name.argtypes = [ctypes.c_char_p]
         L.cs_batched_sort.restype = ctypes.c_int
         L.cs_batched_sort.argtypes = [ctypes.c_int, vp, vp, i64, i32, ctypes.c_int, i32, u32, vp]
+        L.cs_batched_sort_pairs.restype = ctypes.c_int
+        L.cs_batched_sort_pairs.argtypes = [ctypes.c_int, vp, vp, vp, vp, i64, i32, vp]
         L.cs_batched_max_n.restype = i32
         L.cs_batched_max_n.argtypes = [ctypes.c_int]
         L.cs_batched_describe.restype = ctypes.c_int
@@ -160,6 +162,13 @@ def batched_sort(dtype: int, src: int, dst: int, arrays: int, n: int, method: in
     check(lib().cs_batched_sort(dtype, src, dst, arrays, n, method, threads, flags, stream),
           f"cs_batched_sort(dtype={dtype}, n={n}, method={METHODS[method] if 0 <= method < 8 else method}, "
           f"threads={threads}, flags={flags})")
+
+
+def batched_sort_pairs(key_dtype: int, keys_in: int, vals_in: int, keys_out: int, vals_out: int, arrays: int,
+                       n: int, stream: int = 0) -> None:
+    """cs_batched_sort_pairs: keys (I32 or I64) and int32 payloads in two arrays."""
+    check(lib().cs_batched_sort_pairs(key_dtype, keys_in, vals_in, keys_out, vals_out, arrays, n, stream),
+          f"cs_batched_sort_pairs(key_dtype={key_dtype}, n={n})")
 
 
 def batched_status(dtype: int, src: int, dst: int, arrays: int, n: int, method: int = AUTO,

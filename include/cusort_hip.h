@@ -126,6 +126,19 @@ int cs_method_from_name(const char* name);
 int cs_batched_sort(int dtype, const void* in, void* out, int64_t arrays, int32_t n,
                     int method, int32_t threads, uint32_t flags, hipStream_t stream);
 int32_t cs_batched_max_n(int dtype);
+/* Key / payload pairs as two arrays (structure of arrays): keys of
+ * key_dtype (CS_I32 or CS_I64) and int32 payloads, `arrays` arrays of n
+ * packed back to back in each buffer, sorted by key.  No reference
+ * counterpart as an API: it is the SoA form of the IntAndData record
+ * (intanddata.hpp:9-79) and of the int64+int32 record, sorted by the same
+ * network with the same tie rule, so the output equals sorting the records
+ * {key[i], payload[i]} with cs_batched_sort (CS_REC_I32_I32 /
+ * CS_REC_I64_I32, CS_METHOD_AUTO) and splitting them again.  Bytes per
+ * element each way: 8 / 12 (the padded 16-byte record moves 16).
+ * keys_in == keys_out and vals_in == vals_out sort in place.
+ * n <= cs_batched_max_n of the record type (8192 / 4096). */
+int cs_batched_sort_pairs(int key_dtype, const void* keys_in, const int32_t* vals_in, void* keys_out,
+                          int32_t* vals_out, int64_t arrays, int32_t n, hipStream_t stream);
 /* Resources of the kernel cs_batched_sort would launch: threads per
  * workgroup, items per thread, static LDS bytes, VGPRs, arrays per
  * workgroup (the reference reports the same through cudaFuncGetAttributes

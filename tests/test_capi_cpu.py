@@ -73,3 +73,16 @@ def test_null_and_overlap(cs):
     # empty work is a no-op even with null pointers
     assert cs.batched_status(0, 0, 0, 0, 64) == cs.OK
     assert cs.batched_status(0, 0, 0, 4, 0) == cs.OK
+
+
+def test_pairs_arguments(cs):
+    L = cs.lib()
+    assert L.cs_batched_sort_pairs(cs.F32, FAKE, FAKE, FAKE, FAKE, 1, 8, None) == cs.E_INVALID_VALUE
+    assert L.cs_batched_sort_pairs(cs.I32, FAKE, FAKE, FAKE, FAKE, 1, -1, None) == cs.E_INVALID_VALUE
+    assert L.cs_batched_sort_pairs(cs.I32, FAKE, FAKE, FAKE + 4, FAKE, 4, 64, None) == cs.E_INVALID_VALUE
+    assert L.cs_batched_sort_pairs(cs.I32, FAKE, FAKE, FAKE, FAKE, -1, 8, None) == cs.E_INVALID_VALUE
+    assert L.cs_batched_sort_pairs(cs.I32, FAKE, FAKE, FAKE, FAKE, 1, 8193, None) == cs.E_CAPACITY
+    assert L.cs_batched_sort_pairs(cs.I64, FAKE, FAKE, FAKE, FAKE, 1, 4097, None) == cs.E_CAPACITY
+    assert L.cs_batched_sort_pairs(cs.I32, None, FAKE, FAKE, FAKE, 1, 8, None) == cs.E_INVALID_VALUE
+    assert L.cs_batched_sort_pairs(cs.I32, None, None, None, None, 0, 8, None) == cs.OK
+    assert L.cs_batched_sort_pairs(cs.I64, None, None, None, None, 4, 0, None) == cs.OK
