@@ -79,3 +79,27 @@ def test_config4_full(cs, orc, gpu, dtype, n, dist):
     run_large(cs, orc, gpu, dt, getattr(orc, dist), n, "hybrid", ws=ws)
     run_large(cs, orc, gpu, dt, getattr(orc, dist), n, "merge")
     run_large(cs, orc, gpu, dt, getattr(orc, dist), n, "partition")
+
+
+@pytest.mark.parametrize("dtype", ["I32", "F32", "U64", "F64", "REC_I64_I32"])
+def test_partition_level_boundaries(cs, orc, gpu, dtype):
+    """Sizes around the one-level / two-level switch of the partition path
+    (256 parts of 3/8 of the on-chip tile) and small part bounds that force
+    buckets longer than the tile onto the host-planned chunk + merge path."""
+    dt = getattr(cs, dtype)
+    target = (8192 if cs.DTYPE_SIZE[dt] == 4 else 4096) * 3 // 8
+    for n in (256 * target - 1, 256 * target, 256 * target + 1, 300 * target + 17):
+        run_large(cs, orc, gpu, dt, orc.D_UNIFORM, n, "partition", seed=n)
+    run_large(cs, orc, gpu, dt, orc.D_DUP50, 300 * target + 17, "partition", max_parts=4)
+    run_large(cs, orc, gpu, dt, orc.D_UNIFORM, 70000, "partition", min_parts=4000)
+
+
+@pytest.mark.parametrize("dtype", ["F32", "F64"])
+@pytest.mark.parametrize("method", ["partition", "merge"])
+def test_large_special_floats(cs, orc, gpu, dtype, method):
+    """NaNs of both signs, infinities and signed zeros (1 % of the keys)
+    through the large paths: the equality buckets of repeated special values
+    and the totalOrder transform in every classification and merge."""
+    dt = getattr(cs, dtype)
+    for n in (100003, (1 << 21) + 7):
+        run_large(cs, orc, gpu, dt, orc.D_NORMAL_SPECIAL, n, method, seed=n)
