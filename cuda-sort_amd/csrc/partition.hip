@@ -301,7 +301,7 @@ hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, con
     if (e == hipSuccess)
         e = hipMemcpyAsync(segs.p, L.segs.data(), L.segs.size() * sizeof(PartSeg), hipMemcpyHostToDevice, stream);
     const int blocks = int(L.work.size());
-    if (e == hipSuccess) e = pk.count(src, splitters, work.p, blocks, counts.p, stream);
+    if (e == hipSuccess) e = pk.count(src, splitters, work.p, blocks, L.max_ns, counts.p, stream);
     if (e == hipSuccess) e = exclusive_scan(counts.p, L.cells, sums.p, stream);
     if (e == hipSuccess) e = pk.scatter(src, splitters, work.p, blocks, L.max_ns, counts.p, dst, stream);
     if (e == hipSuccess) {

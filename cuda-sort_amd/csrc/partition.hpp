@@ -30,8 +30,7 @@
 namespace cs {
 
 constexpr int kPartThreads = 256;
-constexpr int kPartMaxSplitters = 2047;          // 2P-1 <= 4095 buckets
-constexpr int kPartMaxBuckets = 2 * (kPartMaxSplitters + 1) - 1;
+constexpr int kPartMaxSplitters = 2047;          // 2P-1 <= 4095 buckets (12-bit bucket ids)
 
 __device__ __forceinline__ uint32_t part_hash(uint32_t x) {   // PCG output permutation of x
     x = x * 747796405u + 2891336453u;
@@ -144,30 +143,42 @@ template <class K> __device__ __forceinline__ PartLutMap part_lut_map(const K* s
     return PartLutMap{base, bits > kPartLutBits ? bits - kPartLutBits : 0};
 }
 
+// lut[x] packs the splitter range of prefix x: low half = number of
+// splitters whose prefix is below x, high half = number whose prefix is at
+// most x (one LDS read per key).  sp must hold two readable entries past ns.
 template <class K>
-__device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint16_t* lut, PartLutMap m) {
-    for (int x = int(threadIdx.x); x < kPartLutSize; x += int(blockDim.x)) {
+__device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint32_t* lut, PartLutMap m) {
+    for (int x = int(threadIdx.x); x < kPartLutSize - 1; x += int(blockDim.x)) {
         int lo = 0, hi = ns;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (part_prefix(sp[mid], m) < uint32_t(x)) lo = mid + 1;
             else hi = mid;
         }
-        lut[x] = uint16_t(lo);
+        int end = lo;
+        while (end < ns && part_prefix(sp[end], m) == uint32_t(x)) ++end;
+        lut[x] = uint32_t(lo) | (uint32_t(end) << 16);
     }
 }
 
 // Bucket of key k: j = number of splitters <= k; a key equal to splitter
-// j-1 goes to the equality bucket 2j-1, any other key to bucket 2j.
+// j-1 goes to the equality bucket 2j-1, any other key to bucket 2j.  Up to
+// two candidate splitters (spread-out keys) are compared without a branch;
+// a longer candidate range (keys crowded under one prefix) is searched.
 template <class E>
-__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, const uint16_t* lut,
+__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, const uint32_t* lut,
                                            PartLutMap m) {
-    const uint32_t x = part_prefix(k, m);
-    int lo = lut[x], hi = lut[x + 1];
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (!(k < sp[mid])) lo = mid + 1;
-        else hi = mid;
+    const uint32_t r = lut[part_prefix(k, m)];
+    int lo = int(r & 0xFFFFu), hi = int(r >> 16);
+    if (hi - lo > 2) {
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (!(k < sp[mid])) lo = mid + 1;
+            else hi = mid;
+        }
+    } else {   // sp[lo], sp[lo + 1] are readable (two entries past ns)
+        const typename E::Key s0 = sp[lo], s1 = sp[lo + 1];
+        lo += int(lo < hi && !(k < s0)) + int(lo + 1 < hi && !(k < s1));
     }
     const typename E::Key below = sp[lo > 0 ? lo - 1 : 0];   // read unconditionally: no branch
     return (lo > 0 && !(below < k)) ? 2 * lo - 1 : 2 * lo;
@@ -175,47 +186,57 @@ __device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::
 
 // Per-workgroup histogram: every thread classifies part_items keys per tile,
 // their loads all in flight together (coalesced across the workgroup).
+// Dynamic LDS of k_part_count for up to ns splitters per workgroup:
+// splitters (+2 readable entries) and the histogram.
+template <int DT> size_t part_count_lds(int ns) {
+    return ((size_t(ns + 2) * sizeof(typename Elem<DT>::Key) + 15) & ~size_t(15)) + size_t(2 * ns + 1) * 4;
+}
+
 template <int DT>
 __global__ void __launch_bounds__(kPartThreads)
 k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
-             int64_t* counts) {
+             int max_ns, int64_t* counts) {
     using E = Elem<DT>;
     constexpr int IT = part_items<DT>();
-    __shared__ typename E::Key sp[kPartMaxSplitters];
-    __shared__ int hist[kPartMaxBuckets];
-    __shared__ uint16_t lut[kPartLutSize];
+    extern __shared__ __align__(16) unsigned char part_lds[];
+    typename E::Key* sp = reinterpret_cast<typename E::Key*>(part_lds);
+    int* hist = reinterpret_cast<int*>(part_lds + ((size_t(max_ns + 2) * sizeof(typename E::Key) + 15) & ~size_t(15)));
+    __shared__ uint32_t lut[kPartLutSize - 1];
     const PartBlock w = work[blockIdx.x];
     const int ns = w.ns;
     const int nb = 2 * ns + 1;
     const int tid = int(threadIdx.x);
-    for (int i = tid; i < ns; i += kPartThreads) sp[i] = splitters[w.sp0 + i];
+    for (int i = tid; i < ns + 2; i += kPartThreads) sp[i] = i < ns ? splitters[w.sp0 + i] : typename E::Key{};
     for (int i = tid; i < nb; i += kPartThreads) hist[i] = 0;
     const PartLutMap map = part_lut_map(splitters, w.lo, w.hi);
     __syncthreads();
     part_build_lut(sp, ns, lut, map);
     __syncthreads();
-    const int64_t b1 = w.end;
-    constexpr int64_t STEP = int64_t(IT) * kPartThreads;
-    // the next tile's loads are issued before this tile is classified
-    // Loads past the end re-read the last element (unconditional loads keep
-    // the prefetched tile in flight; their results are masked below).
-    const int64_t last = b1 - 1;
+    // 32-bit element indices relative to the workgroup's range (at most
+    // kPartBlockElems long).  The next tile's loads are issued before this
+    // tile is classified; loads past the end re-read the last element
+    // (unconditional loads keep the prefetched tile in flight; their results
+    // are masked below).
+    const typename E::Storage* base = src + w.begin;
+    const int len = int(w.end - w.begin);
+    const int last = len - 1;
+    constexpr int STEP = IT * kPartThreads;
     typename E::Storage cur[IT];
     #pragma unroll
     for (int k = 0; k < IT; ++k) {
-        const int64_t i = w.begin + int64_t(k) * kPartThreads + tid;
-        cur[k] = src[i < b1 ? i : last];
+        const int i = k * kPartThreads + tid;
+        cur[k] = base[i < len ? i : last];
     }
-    for (int64_t t0 = w.begin; t0 < b1; t0 += STEP) {
+    for (int t0 = 0; t0 < len; t0 += STEP) {
         typename E::Storage nxt[IT];
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const int64_t i = t0 + STEP + int64_t(k) * kPartThreads + tid;
-            nxt[k] = src[i < b1 ? i : last];
+            const int i = t0 + STEP + k * kPartThreads + tid;
+            nxt[k] = base[i < len ? i : last];
         }
         #pragma unroll
         for (int k = 0; k < IT; ++k)
-            if (t0 + int64_t(k) * kPartThreads + tid < b1)
+            if (t0 + k * kPartThreads + tid < len)
                 atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
         #pragma unroll
         for (int k = 0; k < IT; ++k) cur[k] = nxt[k];
@@ -232,17 +253,19 @@ constexpr size_t part_align(size_t x) { return (x + 15) & ~size_t(15); }
 template <int DT> size_t part_scatter_lds(int ns) {
     using E = Elem<DT>;
     const size_t nb = size_t(2 * ns + 1);
-    return part_align(size_t(ns > 0 ? ns : 1) * sizeof(typename E::Key)) + part_align(nb * 4) +
-           part_align(nb * 8) + part_align(size_t(part_scatter_tile<DT>()) * sizeof(typename E::Storage)) +
+    return part_align(size_t(ns + 2) * sizeof(typename E::Key)) + part_align(nb * 4) + 2 * part_align(nb * 8) +
+           part_align(size_t(part_scatter_tile<DT>()) * sizeof(typename E::Storage)) +
            part_align(size_t(part_scatter_tile<DT>()) * 2) + 64;
 }
 
-// Scatter.  Per tile of part_tile elements: classify, rank inside the tile
-// with LDS atomics, scan the tile histogram, stage the tile in bucket order
-// in LDS, then write it out with consecutive threads on consecutive
-// positions of each bucket's run (coalesced where the reference's
-// per-element atomics scatter every key to its own address).  max_ns sizes
-// the LDS carve-up (the largest ns of any workgroup of the launch).
+// Scatter.  Per tile of part_scatter_tile elements: classify, rank inside
+// the tile with LDS atomics, scan the tile histogram (which also advances
+// the workgroup's running output position of every bucket), stage the tile
+// in bucket order in LDS, then write it out with consecutive threads on
+// consecutive positions of each bucket's run (coalesced where the
+// reference's per-element atomics scatter every key to its own address).
+// max_ns sizes the LDS carve-up (the largest ns of any workgroup of the
+// launch).
 template <int DT>
 __global__ void __launch_bounds__(part_scatter_threads<DT>())
 k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
@@ -258,17 +281,21 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
     const int ns = w.ns;
     const int nb = 2 * ns + 1;
     const int max_nb = 2 * max_ns + 1;
+    // LDS: splitters (+2 readable entries), tile histogram / offsets, running
+    // output position and this tile's write base (run - offset) per bucket,
+    // the staged tile and the bucket of every staged element
     K* sp = reinterpret_cast<K*>(part_lds);
-    int* toff = reinterpret_cast<int*>(part_lds + part_align(size_t(max_ns > 0 ? max_ns : 1) * sizeof(K)));
+    int* toff = reinterpret_cast<int*>(part_lds + part_align(size_t(max_ns + 2) * sizeof(K)));
     int64_t* run = reinterpret_cast<int64_t*>(reinterpret_cast<unsigned char*>(toff) + part_align(size_t(max_nb) * 4));
-    S* stage = reinterpret_cast<S*>(reinterpret_cast<unsigned char*>(run) + part_align(size_t(max_nb) * 8));
+    int64_t* wbase = run + part_align(size_t(max_nb) * 8) / 8;
+    S* stage = reinterpret_cast<S*>(reinterpret_cast<unsigned char*>(wbase) + part_align(size_t(max_nb) * 8));
     uint16_t* sbkt = reinterpret_cast<uint16_t*>(reinterpret_cast<unsigned char*>(stage) +
                                                  part_align(size_t(TILE) * sizeof(S)));
     __shared__ int wsum[NT / 64];
-    __shared__ uint16_t lut[kPartLutSize];
+    __shared__ uint32_t lut[kPartLutSize - 1];
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < ns; i += NT) sp[i] = splitters[w.sp0 + i];
+    for (int i = tid; i < ns + 2; i += NT) sp[i] = i < ns ? splitters[w.sp0 + i] : K{};
     for (int b = tid; b < nb; b += NT) {
         toff[b] = 0;
         run[b] = offsets[w.cell + int64_t(b) * w.stride];
@@ -278,30 +305,33 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
     part_build_lut(sp, ns, lut, map);
     __syncthreads();
     const int per_thread = (nb + NT - 1) / NT;   // scan chunk of each thread
-    const int64_t b1 = w.end;
-    // the next tile's loads are issued before this tile is processed
-    // (loads past the end re-read the last element, as in the count kernel)
-    const int64_t last = b1 - 1;
+    // 32-bit indices relative to the workgroup's range; the next tile's loads
+    // are issued before this tile is processed (loads past the end re-read
+    // the last element, as in the count kernel)
+    const S* base = src + w.begin;
+    const int len = int(w.end - w.begin);
+    const int last = len - 1;
     S elem[IT];
     #pragma unroll
     for (int k = 0; k < IT; ++k) {
-        const int64_t i = w.begin + int64_t(k) * NT + tid;
-        elem[k] = src[i < b1 ? i : last];
+        const int i = k * NT + tid;
+        elem[k] = base[i < len ? i : last];
     }
-    for (int64_t t0 = w.begin; t0 < b1; t0 += TILE) {
-        const int tn = int(b1 - t0 < TILE ? b1 - t0 : TILE);
+    for (int t0 = 0; t0 < len; t0 += TILE) {
+        const int tn = len - t0 < TILE ? len - t0 : TILE;
         S nxt[IT];
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
-            const int64_t i = t0 + TILE + int64_t(k) * NT + tid;
-            nxt[k] = src[i < b1 ? i : last];
+            const int i = t0 + TILE + k * NT + tid;
+            nxt[k] = base[i < len ? i : last];
         }
-        int bkt[IT], rank[IT];
+        uint32_t br[IT];   // bucket (12 bits) | rank inside the tile << 12
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
             const int x = k * NT + tid;
-            bkt[k] = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
-            rank[k] = x < tn ? atomicAdd(&toff[bkt[k]], 1) : 0;
+            const int b = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
+            const int r = x < tn ? atomicAdd(&toff[b], 1) : 0;
+            br[k] = uint32_t(b) | (uint32_t(r) << 12);
         }
         __syncthreads();
         // exclusive scan of the tile histogram in place: a chunk per thread,
@@ -322,26 +352,23 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
         for (int b = c0; b < c0 + per_thread && b < nb; ++b) {
             const int c = toff[b];
             toff[b] = acc;
+            wbase[b] = run[b] - acc;   // staged position j of bucket b goes to wbase[b] + j
+            run[b] += c;
             acc += c;
         }
         __syncthreads();
         #pragma unroll
         for (int k = 0; k < IT; ++k) {
             if (k * NT + tid < tn) {
-                const int pos = toff[bkt[k]] + rank[k];
+                const int b = int(br[k] & 0xFFFu);
+                const int pos = toff[b] + int(br[k] >> 12);
                 stage[pos] = elem[k];
-                sbkt[pos] = uint16_t(bkt[k]);
+                sbkt[pos] = uint16_t(b);
             }
         }
         __syncthreads();
-        for (int j = tid; j < tn; j += NT) {
-            const int b = sbkt[j];
-            dst[run[b] + (j - toff[b])] = stage[j];
-        }
-        __syncthreads();
-        for (int b = tid; b < nb; b += NT) run[b] += (b + 1 < nb ? toff[b + 1] : tn) - toff[b];
-        __syncthreads();   // every toff[b + 1] read before any is cleared
-        for (int b = tid; b < nb; b += NT) toff[b] = 0;
+        for (int j = tid; j < tn; j += NT) dst[wbase[sbkt[j]] + j] = stage[j];
+        for (int b = tid; b < nb; b += NT) toff[b] = 0;   // offsets were last read before the barrier above
         #pragma unroll
         for (int k = 0; k < IT; ++k) elem[k] = nxt[k];
         __syncthreads();
@@ -415,8 +442,8 @@ k_seg_merge_pass(const typename Elem<DT>::Storage* src, typename Elem<DT>::Stora
 struct PartKernels {
     hipError_t (*sample)(const void* src, int64_t n, void* samples, int m, hipStream_t s) = nullptr;
     hipError_t (*splitters)(const void* sorted_samples, int m, int parts, void* splitters, hipStream_t s) = nullptr;
-    hipError_t (*count)(const void* src, const void* splitters, const PartBlock* work, int blocks, int64_t* counts,
-                        hipStream_t s) = nullptr;
+    hipError_t (*count)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
+                        int64_t* counts, hipStream_t s) = nullptr;
     hipError_t (*scatter)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
                           const int64_t* offsets, void* dst, hipStream_t s) = nullptr;
     hipError_t (*gather)(const void* global, int parts1, int parts2, void* level1, hipStream_t s) = nullptr;
@@ -444,10 +471,10 @@ PartKernels part_kernels() {
                            static_cast<const S*>(sorted), m, parts, static_cast<K*>(sp));
         return hipGetLastError();
     };
-    k.count = [](const void* src, const void* sp, const PartBlock* work, int blocks, int64_t* counts,
+    k.count = [](const void* src, const void* sp, const PartBlock* work, int blocks, int max_ns, int64_t* counts,
                  hipStream_t s) {
-        hipLaunchKernelGGL(k_part_count<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), 0, s,
-                           static_cast<const S*>(src), static_cast<const K*>(sp), work, counts);
+        hipLaunchKernelGGL(k_part_count<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), part_count_lds<DT>(max_ns),
+                           s, static_cast<const S*>(src), static_cast<const K*>(sp), work, max_ns, counts);
         return hipGetLastError();
     };
     k.scatter = [](const void* src, const void* sp, const PartBlock* work, int blocks, int max_ns,
