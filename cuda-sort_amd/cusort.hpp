@@ -243,6 +243,16 @@ void SortBatched(const T* in, T* out, int n, int64_t arrays, int method = CS_MET
     check(cs_batched_sort(DType<T>::value, in, out, arrays, n, method, threads, flag_bits, stream), "SortBatched");
 }
 
+// Keys and int32 payloads in two arrays, `arrays` arrays of n each (no
+// counterpart in the reference: the structure-of-arrays form of IntAndData
+// and of the int64 + int32 record; same order and tie rule as sorting those
+// records).  K is int or a 64-bit signed integer.
+template <typename K>
+void SortPairs_batched(K* keys, int* vals, int n, int64_t arrays, hipStream_t stream = nullptr) {
+    static_assert(DType<K>::value == CS_I32 || DType<K>::value == CS_I64, "int32 or int64 keys");
+    check(cs_batched_sort_pairs(DType<K>::value, keys, vals, keys, vals, arrays, n, stream), "SortPairs_batched");
+}
+
 }  // namespace cusort_hip
 
 #endif

@@ -106,6 +106,30 @@ int main() {
         expect_sorted_equal(h, download(d, h.size()), n, "HybridSortPartitionedV3 u64");
         (void)hipFree(d);
     }
+    {   // key / payload pairs in two arrays: each pair must survive, keys in order
+        const int n = 1000, arrays = 40;
+        std::vector<long long> hk(static_cast<size_t>(n) * arrays);
+        std::vector<int> hv(hk.size());
+        for (size_t i = 0; i < hk.size(); ++i) {
+            hk[i] = static_cast<long long>(rng() % 50) - 25;
+            hv[i] = int(i);
+        }
+        long long* dk = upload(hk);
+        int* dv = upload(hv);
+        cusort_hip::SortPairs_batched<long long>(dk, dv, n, arrays);
+        const std::vector<long long> gk = download(dk, hk.size());
+        const std::vector<int> gv = download(dv, hv.size());
+        bool good = true;
+        for (size_t i = 0; i < gk.size() && good; ++i) {
+            good = gv[i] >= 0 && size_t(gv[i]) < hk.size() && hk[size_t(gv[i])] == gk[i] &&
+                   size_t(gv[i]) / size_t(n) == i / size_t(n);
+            if (good && i % size_t(n) != 0) good = gk[i - 1] <= gk[i];
+        }
+        std::printf("%s SortPairs_batched int64\n", good ? "PASS" : "FAIL");
+        if (!good) ++failures;
+        (void)hipFree(dk);
+        (void)hipFree(dv);
+    }
     {   // fp64 keys (finite values: operator< agrees with totalOrder up to -0 == +0)
         std::normal_distribution<double> normal(0.0, 1e3);
         for (int n : {1000, 200000}) {
