@@ -326,10 +326,12 @@ hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, con
 // one partition level (data -> tmp) for up to kPartOneLevel parts, else two
 // (data -> tmp -> out) with p1 x p2 parts taken from one global splitter
 // list, so the second level needs no sampling of its own.  The buckets are
-// then sorted as one ragged batch (tmp -> out, or in place in out); the
-// rare bucket longer than a tile is cut into chunks merged by tiled passes.
-// Host synchronisations: bucket starts after each level, and before the
-// one-shot segment plan is released.
+// then sorted as one ragged batch (tmp -> out, or in place in out), planned
+// on the device; the rare bucket longer than a tile sends the call to a
+// host-planned batch that cuts it into chunks merged by tiled passes.
+// Host synchronisations: first-level bucket starts (two levels only) and
+// the final plan's header; the host-planned path reads all bucket starts
+// and synchronises once more at its end.
 int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_t tmp_elems, int min_parts,
                    int max_parts, hipStream_t stream) {
     const LargeKernels lk = large_kernels_for(dtype);
