@@ -35,8 +35,14 @@ constexpr int max_logw(int dtype) {
 // against 14 at E = 16), and every transpose of W <= 2048 is free of LDS
 // bank conflicts under the x + (x >> 5) padding (32 banks served per
 // 32-lane half of the wave).
+//
+// 8-byte elements from W = 4096 up take 16 per lane: with 32, a workgroup's
+// two double-buffered arrays need 139 KB of LDS and only one workgroup fits
+// a CU (measured, 4M elements: {int32, int32} records at W = 4096 56.9 ->
+// 49.2 us, at W = 8192 64.0 -> 61.6 us).  16-byte records always take 16.
 constexpr int auto_loge(int dtype, int logw) {
-    const int cap = dtype == CS_REC_I64_I32 ? 4 : 5;
+    const bool wide8 = dtype == CS_U64 || dtype == CS_I64 || dtype == CS_F64 || dtype == CS_REC_I32_I32;
+    const int cap = (dtype == CS_REC_I64_I32 || (wide8 && logw >= 12)) ? 4 : 5;
     return logw < cap ? logw : cap;
 }
 

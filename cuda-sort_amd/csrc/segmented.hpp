@@ -43,9 +43,9 @@ template <int DT> constexpr int seg_max_logw() {
 }
 
 // Tiling (log2 E) of a register-tile variant for bucket b; -1 if absent.
+// Variant 0 is the batched sort's default tiling.
 template <int DT> constexpr int seg_loge(int b, int variant) {
-    const int cap = DT == CS_REC_I64_I32 ? 4 : 5;
-    if (variant == 0) return b < cap ? b : cap;
+    if (variant == 0) return auto_loge(DT, b);
     if (variant == 1) return b < 4 ? -1 : 4;     // distinct from variant 0 only when b >= 5
     return -1;
 }
@@ -76,8 +76,10 @@ template <class E> constexpr int tile_lds_words(int logw, int loge) {
 // Largest bucket of the single-wave kernel: variant 0 tiles of at most 64
 // lanes.  The multi-wave kernel takes the buckets above it.
 template <int DT> constexpr int seg_small_max() {
-    const int b = seg_loge<DT>(seg_max_logw<DT>(), 0) + 6;
-    return b < seg_max_logw<DT>() ? b : seg_max_logw<DT>() - 1;
+    int m = 0;
+    for (int b = 0; b < seg_max_logw<DT>(); ++b)
+        if (b - seg_loge<DT>(b, 0) <= 6) m = b;
+    return m;
 }
 
 // LDS words (keys; payload uses the same count of int32) a persistent
