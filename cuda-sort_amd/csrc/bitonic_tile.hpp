@@ -88,19 +88,6 @@ struct Tile {
     template <bool kDesc>
     __device__ __forceinline__ void cx(int a, int b) { cmpx<E, kDesc>(k[a], p[a], k[b], p[b]); }
 
-    // Stages 1..e (each lane's slots sorted ascending) by Batcher's odd-even
-    // merge sort: 191 instead of 240 compare-exchanges for 32 slots.  Only
-    // for key-only tiles, whose sorted output does not depend on which
-    // network produced it.
-    __device__ __forceinline__ void presort() {
-        static_assert(!E::kPayload, "records keep the reference's exchanges");
-        constexpr CePairs P = OemOf<NE>::value;
-        sfor<0, P.n>([&](auto c_) {
-            constexpr int c = decltype(c_)::value;
-            cx<false>(P.a[c], P.b[c]);
-        });
-    }
-
     template <int I>
     __device__ __forceinline__ void run(int t, Key* lk, int32_t* lp) {
         if constexpr (I < PlanOf<LOGW, LOGE>::value.nops) {
@@ -310,13 +297,8 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
     }
     if constexpr (E::kPayload && C::NTRANS > 0) tile.template io_transpose<VS, true>(t, lk, lp, kBufFirst);
 
-    // ---- network (key-only tiles: stages 1..e by the shorter odd-even merge network)
-    if constexpr (!E::kPayload && NE >= 4) {
-        tile.presort();
-        tile.template run<in_lane_ops(LOGW, LOGE)>(t, lk, lp);
-    } else {
-        tile.template run<0>(t, lk, lp);
-    }
+    // ---- network
+    tile.template run<0>(t, lk, lp);
 
     // ---- store, lane-striped
     if constexpr (C::NTRANS > 0) tile.template io_transpose<VS, false>(t, lk, lp, kBufLast);

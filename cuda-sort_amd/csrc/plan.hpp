@@ -78,57 +78,6 @@ template <int LOGW, int LOGE> struct PlanOf {
     static constexpr Plan value = make_plan(LOGW, LOGE);
 };
 
-// Number of leading plan ops that sort each lane's slots in place: stages
-// 1..e, in the identity layout, before the first transpose.
-constexpr int in_lane_ops(int w, int e) { return (w < e ? w : e) * ((w < e ? w : e) + 1) / 2; }
-
-// Batcher's odd-even merge sort of n = 2^m slots (ascending): 63 compare-
-// exchanges for 16 slots and 191 for 32, against the 80 / 240 of the
-// bitonic stages 1..m it can stand in for.  Only where ties carry no
-// payload may it replace them: it sorts to the same keys, but not through
-// the same exchanges.
-struct CePairs {
-    int a[192] = {};
-    int b[192] = {};
-    int n = 0;
-};
-
-constexpr void oem_merge(CePairs& p, int lo, int hi, int r) {   // hi inclusive
-    const int step = r * 2;
-    if (step < hi - lo) {
-        oem_merge(p, lo, hi, step);
-        oem_merge(p, lo + r, hi, step);
-        for (int i = lo + r; i < hi - r; i += step) {
-            p.a[p.n] = i;
-            p.b[p.n] = i + r;
-            ++p.n;
-        }
-    } else {
-        p.a[p.n] = lo;
-        p.b[p.n] = lo + r;
-        ++p.n;
-    }
-}
-
-constexpr void oem_sort(CePairs& p, int lo, int hi) {   // hi inclusive
-    if (hi - lo >= 1) {
-        const int mid = lo + (hi - lo) / 2;
-        oem_sort(p, lo, mid);
-        oem_sort(p, mid + 1, hi);
-        oem_merge(p, lo, hi, 1);
-    }
-}
-
-constexpr CePairs make_oem(int n) {
-    CePairs p{};
-    if (n > 1 && n <= 32) oem_sort(p, 0, n - 1);
-    return p;
-}
-
-template <int N> struct OemOf {
-    static constexpr CePairs value = make_oem(N);
-};
-
 // Logical index of slot 0 of lane t in layout L(h) (e slot bits): the low
 // h-e+1 lane bits keep their logical bit positions, the rest sit above h.
 CS_HD constexpr int layout_base(int t, int h, int e) {
