@@ -80,10 +80,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(int64_t* v, int64_t
     }
 }
 
-__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts) {
+__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts, unsigned char* equal) {
     const PartSeg g = segs[blockIdx.x];
-    for (int b = int(threadIdx.x); b < g.nb; b += int(blockDim.x))
+    for (int b = int(threadIdx.x); b < g.nb; b += int(blockDim.x)) {
         starts[g.first + b] = offsets[g.cell0 + int64_t(b) * g.stride];
+        equal[g.first + b] = (g.equal || (b & 1)) ? 1 : 0;
+    }
 }
 
 // ---------------------------------------------------------------- device plan
@@ -92,8 +94,10 @@ __global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64
 // the list holds the bucket ids grouped by class, largest class first, and
 // work items cover `spi` consecutive list entries of one class, as the host
 // planner lays them out (the order inside a class is free: every segment is
-// sorted on its own).  A bucket longer than the tile gets class kPlanLong
-// and sends the call to the host planner, which cuts it into chunks.
+// sorted on its own).  Equality buckets (one key value each) are left as
+// they are when the sort is in place.  A longer bucket than the tile gets
+// class kPlanLong and sends the call to the host planner, which cuts it
+// into chunks.
 constexpr int kPlanClasses = 16;
 constexpr int kPlanLong = 14;
 
@@ -111,16 +115,17 @@ __global__ void k_plan_zero(int* meta) {
     if (threadIdx.x < kPlanMeta) meta[threadIdx.x] = 0;
 }
 
-__global__ void k_plan_classify(int64_t* starts, int64_t nb, int64_t n, int64_t chunk, unsigned char* cls,
-                                int* ccount) {
+__global__ void k_plan_classify(int64_t* starts, const unsigned char* equal, int skip_equal, int64_t nb, int64_t n,
+                                int64_t chunk, unsigned char* cls, int* ccount) {
     __shared__ int h[kPlanClasses];
     if (threadIdx.x < kPlanClasses) h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < nb) {
         const int64_t len = (i + 1 < nb ? starts[i + 1] : n) - starts[i];
-        int c = 255;
-        if (len > chunk) c = kPlanLong;
+        int c = 255;   // empty, or an equality bucket sorted in place: nothing to do
+        if (skip_equal && equal[i]) c = 255;
+        else if (len > chunk) c = kPlanLong;
         else if (len > 0) c = len > 1 ? 64 - __clzll(uint64_t(len - 1)) : 0;
         cls[i] = static_cast<unsigned char>(c);
         if (c < kPlanClasses) atomicAdd(&h[c], 1);
@@ -271,7 +276,7 @@ void add_segment(Level& L, int64_t begin, int64_t end, int sp0, int ns, int lo, 
     if (len <= 0) return;
     const int64_t blocks = (len + kPartBlockElems - 1) / kPartBlockElems;
     const int nb = 2 * ns + 1;
-    L.segs.push_back(PartSeg{L.cells, L.nbuckets, int32_t(blocks), nb});
+    L.segs.push_back(PartSeg{L.cells, L.nbuckets, int32_t(blocks), nb, equal ? 1 : 0, 0});
     L.seg_end.push_back(end);
     L.seg_equal.push_back(equal ? 1 : 0);
     for (int64_t k = 0; k < blocks; ++k) {
@@ -288,7 +293,7 @@ void add_segment(Level& L, int64_t begin, int64_t end, int sp0, int ns, int lo, 
 // d_starts (nbuckets + 1 entries); with h_starts they are also read back
 // (one host synchronisation).
 hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, const void* splitters, void* dst,
-                     int64_t* d_starts, std::vector<int64_t>* h_starts, hipStream_t stream) {
+                     int64_t* d_starts, unsigned char* d_equal, std::vector<int64_t>* h_starts, hipStream_t stream) {
     Scoped<PartBlock> work(stream);
     Scoped<PartSeg> segs(stream);
     Scoped<int64_t> counts(stream), sums(stream);
@@ -306,7 +311,7 @@ hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, con
     if (e == hipSuccess) e = pk.scatter(src, splitters, work.p, blocks, L.max_ns, counts.p, dst, stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_part_starts, dim3(unsigned(L.segs.size())), dim3(256), 0, stream, segs.p, counts.p,
-                           d_starts);
+                           d_starts, d_equal);
         e = hipGetLastError();
     }
     if (h_starts) {
@@ -389,9 +394,12 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
     add_segment(L1, 0, n, 0, p1 - 1, -1, -1, false);
     std::vector<int64_t>& st1 = host.st1;
     Scoped<int64_t> d_st1(stream), d_st2(stream);
+    Scoped<unsigned char> d_eq1(stream), d_eq2(stream);
     e = d_st1.alloc(size_t(L1.nbuckets) + 1);
+    if (e == hipSuccess) e = d_eq1.alloc(size_t(L1.nbuckets));
     if (e == hipSuccess)
-        e = run_level(pk, L1, data, p2 > 1 ? l1sp.p : gsp.p, tmp, d_st1.p, p2 > 1 ? &st1 : nullptr, stream);
+        e = run_level(pk, L1, data, p2 > 1 ? l1sp.p : gsp.p, tmp, d_st1.p, d_eq1.p, p2 > 1 ? &st1 : nullptr,
+                      stream);
     if (e != hipSuccess) return int(e);
 
     // level 2: tmp -> out, one segment per first-level bucket; equality
@@ -413,13 +421,15 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
             }
         }
         e = d_st2.alloc(size_t(L2.nbuckets) + 1);
-        if (e == hipSuccess) e = run_level(pk, L2, tmp, gsp.p, out, d_st2.p, nullptr, stream);
+        if (e == hipSuccess) e = d_eq2.alloc(size_t(L2.nbuckets));
+        if (e == hipSuccess) e = run_level(pk, L2, tmp, gsp.p, out, d_st2.p, d_eq2.p, nullptr, stream);
         if (e != hipSuccess) return int(e);
         fin = &L2;
         fst = &st2;
         src = out;
     }
     int64_t* d_fst = p2 > 1 ? d_st2.p : d_st1.p;
+    const unsigned char* d_feq = p2 > 1 ? d_eq2.p : d_eq1.p;
     const int64_t fnb = fin->nbuckets;
 
     // plan the final ragged sort on the device; only its header comes back
@@ -443,7 +453,10 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
         if (e != hipSuccess) return int(e);
         const unsigned g = unsigned((fnb + 255) / 256);
         hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(128), 0, stream, meta.p);
-        hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, fnb, n, lk.chunk, cls.p, meta.p);
+        // equality buckets need no sort, but an out-of-place final sort must still move them
+        const int skip_equal = src == out ? 1 : 0;
+        hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, d_feq, skip_equal, fnb, n, lk.chunk,
+                           cls.p, meta.p);
         hipLaunchKernelGGL(k_plan_layout, dim3(1), dim3(64), 0, stream, meta.p, t);
         hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls.p, fnb, meta.p, d_list.p);
         hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta.p, t, d_items.p);

@@ -93,6 +93,8 @@ struct PartSeg {
     int64_t first;
     int32_t stride;
     int32_t nb;
+    int32_t equal;   // every bucket of the segment holds one key value (a first-level equality bucket)
+    int32_t pad;
 };
 
 // Classification.  A lookup table indexed by kPartLutBits bits of the key's
@@ -184,6 +186,28 @@ __device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::
     return (lo > 0 && !(below < k)) ? 2 * lo - 1 : 2 * lo;
 }
 
+// Adds `one` (1 per valid lane) to counter[b] for every valid lane and
+// returns the lane's previous value plus its rank among the wave's lanes of
+// the same bucket.  When every valid lane of the wave has the same bucket
+// (sorted or heavily repeated input) one atomic serves the whole wave:
+// per-lane atomics on one address would serialise.
+__device__ __forceinline__ int part_wave_add(int* counter, int b, bool valid) {
+    const int lane = int(__lane_id());
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return 0;
+    // lane reads with a wave-uniform index: v_readlane, no LDS traffic
+    const int first = __ffsll(static_cast<unsigned long long>(vmask)) - 1;
+    const int b0 = __builtin_amdgcn_readlane(b, first);
+    const uint64_t smask = __ballot(valid && b == b0);
+    if (smask == vmask) {   // wave-uniform branch
+        int base = 0;
+        if (lane == first) base = atomicAdd(&counter[b0], __popcll(vmask));
+        base = __builtin_amdgcn_readlane(base, first);
+        return base + __popcll(vmask & ((uint64_t(1) << lane) - 1));
+    }
+    return valid ? atomicAdd(&counter[b], 1) : 0;
+}
+
 // Per-workgroup histogram: every thread classifies part_items keys per tile,
 // their loads all in flight together (coalesced across the workgroup).
 // Dynamic LDS of k_part_count for up to ns splitters per workgroup:
@@ -202,10 +226,12 @@ k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key
     typename E::Key* sp = reinterpret_cast<typename E::Key*>(part_lds);
     int* hist = reinterpret_cast<int*>(part_lds + ((size_t(max_ns + 2) * sizeof(typename E::Key) + 15) & ~size_t(15)));
     __shared__ uint32_t lut[kPartLutSize - 1];
+    __shared__ int crowded;   // some bucket drew a quarter of the first tile
     const PartBlock w = work[blockIdx.x];
     const int ns = w.ns;
     const int nb = 2 * ns + 1;
     const int tid = int(threadIdx.x);
+    if (tid == 0) crowded = 0;
     for (int i = tid; i < ns + 2; i += kPartThreads) sp[i] = i < ns ? splitters[w.sp0 + i] : typename E::Key{};
     for (int i = tid; i < nb; i += kPartThreads) hist[i] = 0;
     const PartLutMap map = part_lut_map(splitters, w.lo, w.hi);
@@ -221,6 +247,7 @@ k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key
     const int len = int(w.end - w.begin);
     const int last = len - 1;
     constexpr int STEP = IT * kPartThreads;
+    bool agg = false;
     typename E::Storage cur[IT];
     #pragma unroll
     for (int k = 0; k < IT; ++k) {
@@ -234,12 +261,28 @@ k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key
             const int i = t0 + STEP + k * kPartThreads + tid;
             nxt[k] = base[i < len ? i : last];
         }
-        #pragma unroll
-        for (int k = 0; k < IT; ++k)
-            if (t0 + k * kPartThreads + tid < len)
-                atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
+        if (agg) {   // workgroup-uniform: sorted or repeated keys crowd one counter
+            #pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const bool valid = t0 + k * kPartThreads + tid < len;
+                const int b = valid ? part_bucket<E>(E::key(cur[k]), sp, lut, map) : 0;
+                (void)part_wave_add(hist, b, valid);
+            }
+        } else {
+            #pragma unroll
+            for (int k = 0; k < IT; ++k)
+                if (t0 + k * kPartThreads + tid < len)
+                    atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
+        }
         #pragma unroll
         for (int k = 0; k < IT; ++k) cur[k] = nxt[k];
+        if (t0 == 0) {   // after the first tile: switch to wave-aggregated counting if crowded
+            __syncthreads();
+            for (int b = tid; b < nb; b += kPartThreads)
+                if (hist[b] >= STEP / 4) crowded = 1;
+            __syncthreads();
+            agg = crowded != 0;
+        }
     }
     __syncthreads();
     for (int b = tid; b < nb; b += kPartThreads) counts[w.cell + int64_t(b) * w.stride] = hist[b];
@@ -293,8 +336,10 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
                                                  part_align(size_t(TILE) * sizeof(S)));
     __shared__ int wsum[NT / 64];
     __shared__ uint32_t lut[kPartLutSize - 1];
+    __shared__ int crowded;   // a bucket drew a quarter of some tile: rank with wave-aggregated atomics
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) crowded = 0;
     for (int i = tid; i < ns + 2; i += NT) sp[i] = i < ns ? splitters[w.sp0 + i] : K{};
     for (int b = tid; b < nb; b += NT) {
         toff[b] = 0;
@@ -326,12 +371,22 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
             nxt[k] = base[i < len ? i : last];
         }
         uint32_t br[IT];   // bucket (12 bits) | rank inside the tile << 12
-        #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-            const int x = k * NT + tid;
-            const int b = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
-            const int r = x < tn ? atomicAdd(&toff[b], 1) : 0;
-            br[k] = uint32_t(b) | (uint32_t(r) << 12);
+        if (crowded) {   // workgroup-uniform (set before the last barrier)
+            #pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int x = k * NT + tid;
+                const int b = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
+                const int r = part_wave_add(toff, b, x < tn);
+                br[k] = uint32_t(b) | (uint32_t(r) << 12);
+            }
+        } else {
+            #pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int x = k * NT + tid;
+                const int b = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
+                const int r = x < tn ? atomicAdd(&toff[b], 1) : 0;
+                br[k] = uint32_t(b) | (uint32_t(r) << 12);
+            }
         }
         __syncthreads();
         // exclusive scan of the tile histogram in place: a chunk per thread,
@@ -351,6 +406,7 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
         for (int q = 0; q < wave; ++q) acc += wsum[q];
         for (int b = c0; b < c0 + per_thread && b < nb; ++b) {
             const int c = toff[b];
+            if (c >= TILE / 4) crowded = 1;   // read by the next tile, after three barriers
             toff[b] = acc;
             wbase[b] = run[b] - acc;   // staged position j of bucket b goes to wbase[b] + j
             run[b] += c;
@@ -375,8 +431,10 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
     }
 }
 
-// Bucket starts of every segment (one workgroup per segment).
-__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts);
+// Bucket starts of every segment (one workgroup per segment), and whether
+// each bucket is an equality bucket (every key in it equal: already sorted,
+// tie order being free).
+__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts, unsigned char* equal);
 
 // The second level's splitters of first-level bucket j are every parts2-th
 // global splitter: first-level splitter j = global splitter (j+1)*parts2 - 1.
