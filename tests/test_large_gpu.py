@@ -103,3 +103,26 @@ def test_large_special_floats(cs, orc, gpu, dtype, method):
     dt = getattr(cs, dtype)
     for n in (100003, (1 << 21) + 7):
         run_large(cs, orc, gpu, dt, orc.D_NORMAL_SPECIAL, n, method, seed=n)
+
+
+def test_repeated_sorts_keep_memory_bounded(cs, orc, gpu):
+    """The stream-ordered temporaries of the partition and merge paths come
+    from a pool that keeps freed memory cached: repeated sorts reuse it, so
+    device memory in use settles after the first calls instead of growing."""
+    n = (1 << 22) + 13
+    host = orc.generate(cs.I32, orc.D_UNIFORM, n, n, 3)
+    buf = cs.from_numpy(host)
+    for _ in range(2):
+        cs.sort_partition(cs.I32, buf.ptr, n)
+        cs.sort_merge(cs.I32, buf.ptr, n)
+    gpu.synchronize()
+    free0, _ = gpu.mem_info()
+    for _ in range(10):
+        buf.upload(host)
+        cs.sort_partition(cs.I32, buf.ptr, n)
+        cs.sort_merge(cs.I32, buf.ptr, n)
+    gpu.synchronize()
+    free1, _ = gpu.mem_info()
+    assert free0 - free1 < (64 << 20), (free0, free1)
+    got = cs.to_numpy(buf, cs.I32, n)
+    assert got.tobytes() == np.sort(host).tobytes()
