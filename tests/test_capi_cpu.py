@@ -86,3 +86,13 @@ def test_pairs_arguments(cs):
     assert L.cs_batched_sort_pairs(cs.I32, None, FAKE, FAKE, FAKE, 1, 8, None) == cs.E_INVALID_VALUE
     assert L.cs_batched_sort_pairs(cs.I32, None, None, None, None, 0, 8, None) == cs.OK
     assert L.cs_batched_sort_pairs(cs.I64, None, None, None, None, 4, 0, None) == cs.OK
+
+
+def test_hybrid_thresholds(cs):
+    # measured merge / partition crossovers: 2^25 for 4-byte, 2^24 for 8-byte,
+    # 2^22 for 16-byte elements; 0 for an unknown element type
+    want = {cs.I32: 25, cs.U32: 25, cs.F32: 25, cs.U64: 24, cs.I64: 24, cs.F64: 24, cs.REC_I32_I32: 24,
+            cs.REC_I64_I32: 22}
+    for dt, lg in want.items():
+        assert cs.hybrid_threshold(dt) == 1 << lg, dt
+    assert cs.hybrid_threshold(99) == 0
