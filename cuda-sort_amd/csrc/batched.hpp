@@ -5,6 +5,7 @@
 
 #include "bitonic_lds.hpp"
 #include "bitonic_tile.hpp"
+#include "bitonic_xlane.hpp"
 
 namespace cs {
 
@@ -70,6 +71,29 @@ constexpr bool tile_exists(int logw, int loge) {
     return (DT == CS_I32 || DT == CS_REC_I32_I32) && loge <= 4;
 }
 
+// Key-only 4-byte element types sort 32..1024-element arrays with the
+// cross-lane register engine (bitonic_xlane.hpp) instead of the LDS-transpose
+// tile of the same shape (32 keys per lane, W / 32 lanes per array).  Stages
+// from kXlaneWindow on move through LDS windows: at W = 1024 that keeps lane
+// positions 4 and 5 (permlane swaps, three issue slots per element) out of
+// the network and leaves four LDS passes instead of the tile's ten.
+constexpr int kXlaneWindow = 9;
+constexpr bool xlane_dtype(int dt) { return dt == CS_I32 || dt == CS_U32 || dt == CS_F32; }
+constexpr bool xlane_shape(int dt, int logw, int loge) {
+    return xlane_dtype(dt) && loge == xl::kLogE && logw >= xl::kLogE && logw <= 10;
+}
+
+template <int DT, int LW>
+hipError_t launch_xlane(const void* in, void* out, int64_t arrays, int n, int, unsigned, int, hipStream_t s) {
+    using S = typename Elem<DT>::Storage;
+    constexpr int apb = 4 * (64 >> (LW - xl::kLogE));
+    const int64_t blocks = (arrays + apb - 1) / apb;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_bitonic_xlane<Elem<DT>, LW, kXlaneWindow>), dim3(unsigned(blocks)), dim3(256), 0, s,
+                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n);
+    return hipGetLastError();
+}
+
 template <int DT, int LW, int LE>
 hipError_t launch_tile(const void* in, void* out, int64_t arrays, int n, int, unsigned, int, hipStream_t s) {
     using C = TileCfg<Elem<DT>, LW, LE>;
@@ -119,7 +143,16 @@ KernelInfo tile_lookup(int logw, int loge) {
         constexpr int LW = decltype(w_)::value;
         hfor<0, 6>([&](auto e_) {
             constexpr int LE = decltype(e_)::value;
-            if constexpr (tile_exists<DT>(LW, LE)) {
+            if constexpr (xlane_shape(DT, LW, LE)) {
+                if (LW == logw && LE == loge) {
+                    r.launch = &launch_xlane<DT, LW>;
+                    r.kernel = reinterpret_cast<const void*>(&k_bitonic_xlane<Elem<DT>, LW, kXlaneWindow>);
+                    r.block = 256;
+                    r.items = xl::kE;
+                    r.apb = 4 * (64 >> (LW - xl::kLogE));
+                    r.lds = r.apb * xl::region_words(LW) * 4;
+                }
+            } else if constexpr (tile_exists<DT>(LW, LE)) {
                 if (LW == logw && LE == loge) {
                     using C = TileCfg<Elem<DT>, LW, LE>;
                     r.launch = &launch_tile<DT, LW, LE>;
