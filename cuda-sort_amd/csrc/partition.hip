@@ -9,6 +9,7 @@
 #include "../../include/cusort_hip.h"
 #include "gen.hpp"
 #include "ragged.hpp"
+#include "scratch.hpp"
 
 namespace cs {
 
@@ -221,15 +222,6 @@ hipError_t exclusive_scan(int64_t* v, int64_t n, int64_t* sums, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <class T>
-struct Scoped {   // stream-ordered temporary
-    T* p = nullptr;
-    hipStream_t s;
-    explicit Scoped(hipStream_t st) : s(st) {}
-    hipError_t alloc(size_t count) { return pool_malloc_async(reinterpret_cast<void**>(&p), count * sizeof(T), s); }
-    ~Scoped() { if (p) (void)hipFreeAsync(p, s); }
-};
-
 }  // namespace
 
 namespace {
@@ -292,26 +284,27 @@ void add_segment(Level& L, int64_t begin, int64_t end, int sp0, int ns, int lo, 
 // Count, scan and scatter of one level (src -> dst) and its bucket starts in
 // d_starts (nbuckets + 1 entries); with h_starts they are also read back
 // (one host synchronisation).
-hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, const void* splitters, void* dst,
-                     int64_t* d_starts, unsigned char* d_equal, std::vector<int64_t>* h_starts, hipStream_t stream) {
-    Scoped<PartBlock> work(stream);
-    Scoped<PartSeg> segs(stream);
-    Scoped<int64_t> counts(stream), sums(stream);
-    hipError_t e = work.alloc(L.work.size());
-    if (e == hipSuccess) e = segs.alloc(L.segs.size());
-    if (e == hipSuccess) e = counts.alloc(size_t(L.cells));
-    if (e == hipSuccess) e = sums.alloc(size_t((L.cells + kScanTile - 1) / kScanTile) + 1);
+int run_level(const PartKernels& pk, const Level& L, const void* src, const void* splitters, void* dst,
+              int64_t* d_starts, unsigned char* d_equal, std::vector<int64_t>* h_starts, Scratch& tmp) {
+    const hipStream_t stream = tmp.stream();
+    PartBlock* work = nullptr;
+    PartSeg* segs = nullptr;
+    int64_t *counts = nullptr, *sums = nullptr;
+    int rc = tmp.get(&work, L.work.size());
+    if (rc == CS_OK) rc = tmp.get(&segs, L.segs.size());
+    if (rc == CS_OK) rc = tmp.get(&counts, size_t(L.cells));
+    if (rc == CS_OK) rc = tmp.get(&sums, size_t((L.cells + kScanTile - 1) / kScanTile) + 1);
+    if (rc != CS_OK) return rc;
+    hipError_t e = hipMemcpyAsync(work, L.work.data(), L.work.size() * sizeof(PartBlock), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(work.p, L.work.data(), L.work.size() * sizeof(PartBlock), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(segs.p, L.segs.data(), L.segs.size() * sizeof(PartSeg), hipMemcpyHostToDevice, stream);
+        e = hipMemcpyAsync(segs, L.segs.data(), L.segs.size() * sizeof(PartSeg), hipMemcpyHostToDevice, stream);
     const int blocks = int(L.work.size());
-    if (e == hipSuccess) e = pk.count(src, splitters, work.p, blocks, L.max_ns, counts.p, stream);
-    if (e == hipSuccess) e = exclusive_scan(counts.p, L.cells, sums.p, stream);
-    if (e == hipSuccess) e = pk.scatter(src, splitters, work.p, blocks, L.max_ns, counts.p, dst, stream);
+    if (e == hipSuccess) e = pk.count(src, splitters, work, blocks, L.max_ns, counts, stream);
+    if (e == hipSuccess) e = exclusive_scan(counts, L.cells, sums, stream);
+    if (e == hipSuccess) e = pk.scatter(src, splitters, work, blocks, L.max_ns, counts, dst, stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_part_starts, dim3(unsigned(L.segs.size())), dim3(256), 0, stream, segs.p, counts.p,
-                           d_starts, d_equal);
+        hipLaunchKernelGGL(k_part_starts, dim3(unsigned(L.segs.size())), dim3(256), 0, stream, segs, counts, d_starts,
+                           d_equal);
         e = hipGetLastError();
     }
     if (h_starts) {
@@ -321,71 +314,92 @@ hipError_t run_level(const PartKernels& pk, const Level& L, const void* src, con
                                stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
     }
-    return e;
+    return e == hipSuccess ? CS_OK : int(e);
 }
 
-}  // namespace
+// Parts of a partition sort of n elements (the caller's MinNbParts /
+// MaxNbParts bound them when nonzero): p1 first-level x p2 second-level
+// parts (p2 = 1: one level) and m sampled keys.
+struct PartShape {
+    int64_t target = 0;
+    int p1 = 0, p2 = 1;
+    int64_t parts = 0, m = 0;
+};
 
-// Sorts data[0, n) into out (out may equal data) with tmp as scratch of at
-// least n elements.  Sample sort into buckets of ~3/8 of the on-chip tile:
-// one partition level (data -> tmp) for up to kPartOneLevel parts, else two
-// (data -> tmp -> out) with p1 x p2 parts taken from one global splitter
-// list, so the second level needs no sampling of its own.  The buckets are
-// then sorted as one ragged batch (tmp -> out, or in place in out), planned
-// on the device; the rare bucket longer than a tile sends the call to a
-// host-planned batch that cuts it into chunks merged by tiled passes.
-// Host synchronisations: first-level bucket starts (two levels only) and
-// the final plan's header; the host-planned path reads all bucket starts
-// and synchronises once more at its end.
-int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_t tmp_elems, int min_parts,
-                   int max_parts, hipStream_t stream) {
+PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
     const LargeKernels lk = large_kernels_for(dtype);
-    const PartKernels pk = part_kernels_for(dtype);
     const size_t esz = cs_dtype_size(dtype);
-    if (tmp_elems < size_t(n)) return CS_E_WORKSPACE;
-    if (n <= lk.chunk) return cs_batched_sort(dtype, data, out, 1, int32_t(n), CS_METHOD_AUTO, 0, 0, stream);
-
+    PartShape sh;
     // Buckets average 3/8 of the tile (3072 4-byte / 1536 wider keys): with
     // 16-64 samples per part the largest of ~10^5 buckets stays under the
     // tile and most fill the next power of two to ~75 %.
-    const int64_t target = lk.chunk * 3 / 8;
-    int64_t P = (n + target - 1) / target;
-    if (min_parts > 0) P = std::max<int64_t>(P, min_parts);   // the caller's MinNbParts / MaxNbParts
+    sh.target = lk.chunk * 3 / 8;
+    int64_t P = (n + sh.target - 1) / sh.target;
+    if (min_parts > 0) P = std::max<int64_t>(P, min_parts);
     if (max_parts > 0) P = std::min<int64_t>(P, max_parts);
     const int64_t pmax = int64_t(kPartMaxSplitters + 1) * (kPartMaxSplitters + 1);
     P = std::max<int64_t>(2, std::min(P, pmax));
-    int p1 = int(P), p2 = 1;
+    sh.p1 = int(P);
     if (P > kPartOneLevel) {
-        p1 = int(std::ceil(std::sqrt(double(P))));
-        p2 = int(P / p1);   // p1 * p2 <= P: a caller's part bound holds
-        p1 = std::min(p1, kPartMaxSplitters + 1);
-        p2 = std::min(p2, kPartMaxSplitters + 1);
+        sh.p1 = int(std::ceil(std::sqrt(double(P))));
+        sh.p2 = int(P / sh.p1);   // p1 * p2 <= P: a caller's part bound holds
+        sh.p1 = std::min(sh.p1, kPartMaxSplitters + 1);
+        sh.p2 = std::min(sh.p2, kPartMaxSplitters + 1);
     }
-    const int64_t parts = int64_t(p1) * p2;
+    sh.parts = int64_t(sh.p1) * sh.p2;
     // Samples per part.  Two levels, 2^28 int32: 16 per part 4.97 ms, 32 per
     // part 5.09 ms (a smaller sample to sort outweighs a wider spread of
     // bucket sizes); 2^27 uint64: 5.20 ms against 4.92 (its buckets of 1536
     // spill into the next tile size sooner), so wider keys keep 32.
-    const int64_t m = parts * (p2 > 1 ? (esz == 4 ? 16 : 32) : 64);
+    sh.m = sh.parts * (sh.p2 > 1 ? (esz == 4 ? 16 : 32) : 64);
+    return sh;
+}
 
-    Scoped<unsigned char> samples(stream), samples_tmp(stream), gsp(stream), l1sp(stream);
-    hipError_t e = samples.alloc(size_t(m) * esz);
-    if (e == hipSuccess) e = samples_tmp.alloc(size_t(m) * esz);
-    if (e == hipSuccess) e = gsp.alloc(size_t(parts - 1) * size_t(pk.key_size));
-    if (e == hipSuccess && p2 > 1) e = l1sp.alloc(size_t(std::max(p1 - 1, 1)) * size_t(pk.key_size));
-    if (e != hipSuccess) return int(e);
+}  // namespace
+
+// Sorts data[0, n) into out (out may equal data) with `values` as scratch of
+// at least n elements and the call's other temporaries from `tmp`.  Sample
+// sort into buckets of ~3/8 of the on-chip tile: one partition level
+// (data -> values) for up to kPartOneLevel parts, else two (data -> values ->
+// out) with p1 x p2 parts taken from one global splitter list, so the second
+// level needs no sampling of its own.  The buckets are then sorted as one
+// ragged batch (values -> out, or in place in out), planned on the device;
+// the rare bucket longer than a tile sends the call to a host-planned batch
+// that cuts it into chunks merged by tiled passes.  Host synchronisations:
+// first-level bucket starts (two levels only) and the final plan's header;
+// the host-planned path reads all bucket starts and synchronises once more at
+// its end.
+int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, size_t values_elems, int min_parts,
+                   int max_parts, Scratch& tmp) {
+    const LargeKernels lk = large_kernels_for(dtype);
+    const PartKernels pk = part_kernels_for(dtype);
+    const size_t esz = cs_dtype_size(dtype);
+    const hipStream_t stream = tmp.stream();
+    if (values_elems < size_t(n)) return CS_E_WORKSPACE;
+    if (n <= lk.chunk) return cs_batched_sort(dtype, data, out, 1, int32_t(n), CS_METHOD_AUTO, 0, 0, stream);
+
+    const PartShape sh = part_shape(dtype, n, min_parts, max_parts);
+    const int p1 = sh.p1, p2 = sh.p2;
+    const int64_t parts = sh.parts, m = sh.m, target = sh.target;
+
+    unsigned char *samples = nullptr, *samples_tmp = nullptr, *gsp = nullptr, *l1sp = nullptr;
+    int rc = tmp.get(&samples, size_t(m) * esz);
+    if (rc == CS_OK) rc = tmp.get(&samples_tmp, size_t(m) * esz);
+    if (rc == CS_OK) rc = tmp.get(&gsp, size_t(parts - 1) * size_t(pk.key_size));
+    if (rc == CS_OK && p2 > 1) rc = tmp.get(&l1sp, size_t(std::max(p1 - 1, 1)) * size_t(pk.key_size));
+    if (rc != CS_OK) return rc;
 
     // splitters from a sorted hashed sample; the first level's are every
     // p2-th of them
-    e = pk.sample(data, n, samples.p, int(m), stream);
+    hipError_t e = pk.sample(data, n, samples, int(m), stream);
     if (e != hipSuccess) return int(e);
-    int rc = cs_sort_merge(dtype, samples.p, m, 0, 0, samples_tmp.p, size_t(m), 1, stream);
+    rc = merge_sort(dtype, samples, m, samples, samples_tmp, tmp);
     if (rc != CS_OK) return rc;
-    e = pk.splitters(samples.p, int(m), int(parts), gsp.p, stream);
-    if (e == hipSuccess && p2 > 1) e = pk.gather(gsp.p, p1, p2, l1sp.p, stream);
+    e = pk.splitters(samples, int(m), int(parts), gsp, stream);
+    if (e == hipSuccess && p2 > 1) e = pk.gather(gsp, p1, p2, l1sp, stream);
     if (e != hipSuccess) return int(e);
 
-    // level 1: data -> tmp
+    // level 1: data -> values
     thread_local PartHost host;
     Level& L1 = host.l1;
     Level& L2 = host.l2;
@@ -393,21 +407,20 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
     clear_level(L2);
     add_segment(L1, 0, n, 0, p1 - 1, -1, -1, false);
     std::vector<int64_t>& st1 = host.st1;
-    Scoped<int64_t> d_st1(stream), d_st2(stream);
-    Scoped<unsigned char> d_eq1(stream), d_eq2(stream);
-    e = d_st1.alloc(size_t(L1.nbuckets) + 1);
-    if (e == hipSuccess) e = d_eq1.alloc(size_t(L1.nbuckets));
-    if (e == hipSuccess)
-        e = run_level(pk, L1, data, p2 > 1 ? l1sp.p : gsp.p, tmp, d_st1.p, d_eq1.p, p2 > 1 ? &st1 : nullptr,
-                      stream);
-    if (e != hipSuccess) return int(e);
+    int64_t *d_st1 = nullptr, *d_st2 = nullptr;
+    unsigned char *d_eq1 = nullptr, *d_eq2 = nullptr;
+    rc = tmp.get(&d_st1, size_t(L1.nbuckets) + 1);
+    if (rc == CS_OK) rc = tmp.get(&d_eq1, size_t(L1.nbuckets));
+    if (rc == CS_OK)
+        rc = run_level(pk, L1, data, p2 > 1 ? l1sp : gsp, values, d_st1, d_eq1, p2 > 1 ? &st1 : nullptr, tmp);
+    if (rc != CS_OK) return rc;
 
-    // level 2: tmp -> out, one segment per first-level bucket; equality
+    // level 2: values -> out, one segment per first-level bucket; equality
     // buckets and buckets already small are copied through
     const Level* fin = &L1;
     const std::vector<int64_t>* fst = &st1;
     std::vector<int64_t>& st2 = host.st2;
-    void* src = tmp;
+    void* src = values;
     if (p2 > 1) {
         const int nb1 = 2 * p1 - 1;
         for (int b = 0; b < nb1; ++b) {
@@ -420,16 +433,16 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
                             j < p1 - 1 ? (j + 1) * p2 - 1 : -1, false);
             }
         }
-        e = d_st2.alloc(size_t(L2.nbuckets) + 1);
-        if (e == hipSuccess) e = d_eq2.alloc(size_t(L2.nbuckets));
-        if (e == hipSuccess) e = run_level(pk, L2, tmp, gsp.p, out, d_st2.p, d_eq2.p, nullptr, stream);
-        if (e != hipSuccess) return int(e);
+        rc = tmp.get(&d_st2, size_t(L2.nbuckets) + 1);
+        if (rc == CS_OK) rc = tmp.get(&d_eq2, size_t(L2.nbuckets));
+        if (rc == CS_OK) rc = run_level(pk, L2, values, gsp, out, d_st2, d_eq2, nullptr, tmp);
+        if (rc != CS_OK) return rc;
         fin = &L2;
         fst = &st2;
         src = out;
     }
-    int64_t* d_fst = p2 > 1 ? d_st2.p : d_st1.p;
-    const unsigned char* d_feq = p2 > 1 ? d_eq2.p : d_eq1.p;
+    int64_t* d_fst = p2 > 1 ? d_st2 : d_st1;
+    const unsigned char* d_feq = p2 > 1 ? d_eq2 : d_eq1;
     const int64_t fnb = fin->nbuckets;
 
     // plan the final ragged sort on the device; only its header comes back
@@ -442,37 +455,36 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
             t.spi[c] = c <= sc.maxb ? std::max(1, sc.spi[c]) : 1;
             t.variant[c] = c <= sc.maxb ? sc.variant[c] : 0;
         }
-        Scoped<unsigned char> cls(stream);
-        Scoped<int> meta(stream), d_list(stream), d_counters(stream);
-        Scoped<SegItem> d_items(stream);
-        e = cls.alloc(size_t(fnb));
-        if (e == hipSuccess) e = meta.alloc(kPlanMeta);
-        if (e == hipSuccess) e = d_list.alloc(size_t(fnb));
-        if (e == hipSuccess) e = d_items.alloc(size_t(fnb));
-        if (e == hipSuccess) e = d_counters.alloc(2);
-        if (e != hipSuccess) return int(e);
+        unsigned char* cls = nullptr;
+        int *meta = nullptr, *d_list = nullptr, *d_counters = nullptr;
+        SegItem* d_items = nullptr;
+        rc = tmp.get(&cls, size_t(fnb));
+        if (rc == CS_OK) rc = tmp.get(&meta, kPlanMeta);
+        if (rc == CS_OK) rc = tmp.get(&d_list, size_t(fnb));
+        if (rc == CS_OK) rc = tmp.get(&d_items, size_t(fnb));
+        if (rc == CS_OK) rc = tmp.get(&d_counters, 2);
+        if (rc != CS_OK) return rc;
         const unsigned g = unsigned((fnb + 255) / 256);
-        hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(128), 0, stream, meta.p);
+        hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(128), 0, stream, meta);
         // equality buckets need no sort, but an out-of-place final sort must still move them
         const int skip_equal = src == out ? 1 : 0;
         hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, d_feq, skip_equal, fnb, n, lk.chunk,
-                           cls.p, meta.p);
-        hipLaunchKernelGGL(k_plan_layout, dim3(1), dim3(64), 0, stream, meta.p, t);
-        hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls.p, fnb, meta.p, d_list.p);
-        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta.p, t, d_items.p);
+                           cls, meta);
+        hipLaunchKernelGGL(k_plan_layout, dim3(1), dim3(64), 0, stream, meta, t);
+        hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls, fnb, meta, d_list);
+        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta, t, d_items);
         int hdr[3] = {0, 0, 0};
         e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hdr, meta.p + 4 * kPlanClasses, sizeof(hdr), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hdr, meta + 4 * kPlanClasses, sizeof(hdr), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return int(e);
         if (hdr[2] == 0) {   // no bucket longer than the tile: sort with the device plan
-            SegShape sh;
-            sh.grid[kSegMulti] = seg_persistent_grid(dtype, kSegMulti, 1 << 30);
-            sh.grid[kSegSingle] = seg_persistent_grid(dtype, kSegSingle, 1 << 30);
-            if (sh.grid[kSegMulti] <= 0 || sh.grid[kSegSingle] <= 0) return CS_E_NO_DEVICE;
-            e = seg_launch_all(dtype, sh, src, out, d_fst, d_list.p, d_items.p, 0, d_counters.p, stream,
-                               meta.p + 4 * kPlanClasses);
+            SegShape ss;
+            ss.grid[kSegMulti] = seg_persistent_grid(dtype, kSegMulti, tmp.grid_cap);
+            ss.grid[kSegSingle] = seg_persistent_grid(dtype, kSegSingle, tmp.grid_cap);
+            if (ss.grid[kSegMulti] <= 0 || ss.grid[kSegSingle] <= 0) return CS_E_NO_DEVICE;
+            e = seg_launch_all(dtype, ss, src, out, d_fst, d_list, d_items, 0, d_counters, stream,
+                               meta + 4 * kPlanClasses);
             return e == hipSuccess ? CS_OK : int(e);
         }
     }
@@ -521,40 +533,80 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* tmp, size_
     rc = seg_build_items(dtype, seg.data(), nseg, nullptr, list, items, nullptr, nullptr, nullptr);
     if (rc != CS_OK) return rc;
     const SegShape shape = seg_shape(dtype, items);
-    Scoped<int64_t> d_seg(stream);
-    Scoped<int> d_list(stream), d_counters(stream);
-    Scoped<SegItem> d_items(stream);
-    Scoped<MergeTile> d_tiles(stream);
-    e = d_seg.alloc(seg.size());
-    if (e == hipSuccess) e = d_list.alloc(list.size());
-    if (e == hipSuccess) e = d_items.alloc(items.size());
-    if (e == hipSuccess) e = d_counters.alloc(2);
-    if (e == hipSuccess && !tiles.empty()) e = d_tiles.alloc(tiles.size());
-    if (e == hipSuccess) e = hipMemcpyAsync(d_seg.p, seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_list.p, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, stream);
+    int64_t* d_seg = nullptr;
+    int *d_list = nullptr, *d_counters = nullptr;
+    SegItem* d_items = nullptr;
+    MergeTile* d_tiles = nullptr;
+    rc = tmp.get(&d_seg, seg.size());
+    if (rc == CS_OK) rc = tmp.get(&d_list, list.size());
+    if (rc == CS_OK) rc = tmp.get(&d_items, items.size());
+    if (rc == CS_OK) rc = tmp.get(&d_counters, 2);
+    if (rc == CS_OK && !tiles.empty()) rc = tmp.get(&d_tiles, tiles.size());
+    if (rc != CS_OK) return rc;
+    e = hipMemcpyAsync(d_seg, seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(SegItem), hipMemcpyHostToDevice, stream);
+        e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(SegItem), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess && !tiles.empty())
-        e = hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(MergeTile), hipMemcpyHostToDevice, stream);
+        e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(MergeTile), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess)
-        e = seg_launch_all(dtype, shape, src, out, d_seg.p, d_list.p, d_items.p, int(items.size()), d_counters.p,
-                           stream);
+        e = seg_launch_all(dtype, shape, src, out, d_seg, d_list, d_items, int(items.size()), d_counters, stream);
     rc = e == hipSuccess ? CS_OK : int(e);
-    // merge passes over the long buckets (out <-> tmp), then final placement
+    // merge passes over the long buckets (out <-> values), then final placement
     void* a = out;
-    void* b2 = tmp;
+    void* b2 = values;
     for (int p = 0; rc == CS_OK && p < passes; ++p) {
-        e = pk.seg_merge(a, b2, d_tiles.p, int(tiles.size()), chunk << p, 0, stream);
+        e = pk.seg_merge(a, b2, d_tiles, int(tiles.size()), chunk << p, 0, stream);
         if (e != hipSuccess) rc = int(e);
         std::swap(a, b2);
     }
     if (rc == CS_OK && a != out) {
-        e = pk.seg_merge(a, out, d_tiles.p, int(tiles.size()), 0, 1, stream);
+        e = pk.seg_merge(a, out, d_tiles, int(tiles.size()), 0, 1, stream);
         if (e != hipSuccess) rc = int(e);
     }
     const hipError_t se = hipStreamSynchronize(stream);   // host arrays above are read by the copies
     if (rc == CS_OK && se != hipSuccess) rc = int(se);
     return rc;
+}
+
+// Every temporary partition_sort can take for n elements, at its largest:
+// the data-dependent second level and the host-planned fallback are bounded
+// by their part and chunk counts.
+size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_parts) {
+    const LargeKernels lk = large_kernels_for(dtype);
+    const PartKernels pk = part_kernels_for(dtype);
+    const size_t esz = cs_dtype_size(dtype);
+    if (!lk.chunk || n <= lk.chunk) return 0;
+    const PartShape sh = part_shape(dtype, n, min_parts, max_parts);
+    auto r = [](size_t bytes) { return scratch_round(bytes ? bytes : 1); };
+    size_t total = 0;
+    total += 2 * r(size_t(sh.m) * esz);                                   // samples, sorted copy
+    total += merge_scratch_bytes(dtype, sh.m);                            // their merge sort
+    total += r(size_t(sh.parts - 1) * size_t(pk.key_size));               // global splitters
+    total += r(size_t(std::max(sh.p1 - 1, 1)) * size_t(pk.key_size));     // first-level splitters
+    auto level = [&](int64_t segs, int64_t blocks, int64_t nbuckets, int64_t cells) {
+        size_t b = r(size_t(nbuckets + 1) * 8) + r(size_t(nbuckets));     // starts, equality flags
+        b += r(size_t(blocks) * sizeof(PartBlock)) + r(size_t(segs) * sizeof(PartSeg));
+        b += r(size_t(cells) * 8) + r(size_t((cells + kScanTile - 1) / kScanTile + 1) * 8);
+        return b;
+    };
+    const int64_t blocks1 = (n + kPartBlockElems - 1) / kPartBlockElems;
+    const int64_t nb1 = 2 * int64_t(sh.p1) - 1;
+    total += level(1, blocks1, nb1, nb1 * blocks1);
+    int64_t fnb = nb1;
+    if (sh.p2 > 1) {
+        const int64_t nbs = 2 * int64_t(sh.p2) - 1;                       // buckets per second-level segment
+        const int64_t blocks2 = blocks1 + nb1;                            // each segment rounds up once
+        fnb = nbs * nb1;
+        total += level(nb1, blocks2, fnb, nbs * blocks2);
+    }
+    // device plan of the final sort
+    total += r(size_t(fnb)) + r(size_t(kPlanMeta) * 4) + r(size_t(fnb) * 4) + r(size_t(fnb) * sizeof(SegItem)) + r(8);
+    // host-planned fallback: every bucket plus every chunk of a long one
+    const int64_t segs = fnb + n / lk.chunk + 2;
+    total += r(size_t(segs) * 8) + r(size_t(segs) * 4) + r(size_t(segs) * sizeof(SegItem)) + r(8);
+    total += r(size_t(n / lk.tile + fnb + 1) * sizeof(MergeTile));
+    return total;
 }
 
 }  // namespace cs

@@ -9,7 +9,7 @@
 //   HybridSortPartitionedV3 (2488-2497)              HybridSortPartitionedV3 -> cs_sort_partition
 //   HybridSortLegacy (2355-2356, never called)       HybridSortLegacy       -> cs_sort_hybrid
 //   CuSortWorkspace<T,MaxNbParts> (67-219)           CuSortWorkspace<T,MaxNbParts> (RAII, move-only)
-//   CuSortTuning<T> (2479-2483)                      CuSortTuning<T>        (threshold from the library)
+//   CuSortTuning<T> (2479-2483)                      CuSortTuning<T>        (specialisable constexpr threshold)
 //   batched kernels of bench.cu:355-393, launched    Sort_1Warp_SM_le32_batched, Sort_1Block_batched<Variant>,
 //   kernel<<<arrays, threads, smem>>>(data, n)        Sort_1Block_v4_xor_batched, Sort_1Block_v5_static_batched,
 //                                                    Sort_1Block_v6_register_batched -> cs_batched_sort
@@ -29,6 +29,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "../include/cusort_hip.h"
 
@@ -91,40 +92,133 @@ constexpr uint32_t flags() {
 }
 
 // ---------------------------------------------------------- workspace ----
+// CuSortWorkspace<T, MaxNbParts> (reference cusort.hpp:67-219): RAII and
+// move-only over a cs_workspace.  The library keeps the values buffer, an
+// arena for every other temporary of a sort of up to capacity() elements and
+// nbSortStreams() worker streams with completion events; sorts through a
+// workspace allocate no device memory.  The reference's scratch accessors
+// (pivots, counters, blockOffsets, ...) return regions of the arena sized as
+// the reference's arrays (MaxNbParts entries, persistentBlocks x MaxNbParts
+// for the per-block ones); the library's kernels keep their own data there,
+// so their contents are not the reference's.
 template <typename T, int MaxNbParts = 1024>
 class CuSortWorkspace {
     static_assert(MaxNbParts >= 32 && MaxNbParts <= 1024 && ((MaxNbParts - 1) & MaxNbParts) == 0,
                   "MaxNbParts must be a power of two in [32, 1024]");
     cs_workspace* ws_ = nullptr;
-    int blocks_per_sm_ = 4;
-    int streams_ = 8;
+    std::vector<int> host_counters_ = std::vector<int>(MaxNbParts + 1);
+    std::vector<int> host_uniform_parts_ = std::vector<int>(MaxNbParts);
+    std::vector<int> host_large_partitions_ = std::vector<int>(MaxNbParts);
+
+    template <class P>
+    P* arena_at(size_t byte_offset) const {
+        void* arena = nullptr;
+        size_t bytes = 0;
+        cs_workspace_info(ws_, nullptr, &arena, &bytes, nullptr, nullptr);
+        return arena && byte_offset < bytes ? reinterpret_cast<P*>(static_cast<char*>(arena) + byte_offset) : nullptr;
+    }
+    // Byte offsets of the reference-shaped regions inside the arena.
+    size_t off_counters() const { return size_t(MaxNbParts) * sizeof(T); }
+    size_t off_block_offsets() const { return off_counters() + size_t(MaxNbParts + 2) * sizeof(int); }
+    size_t off_block_uniform() const {
+        return off_block_offsets() + size_t(persistentBlocks()) * MaxNbParts * sizeof(int);
+    }
+    size_t off_uniform_parts() const {
+        return off_block_uniform() + size_t(persistentBlocks()) * MaxNbParts * sizeof(int);
+    }
+    size_t reference_regions() const { return off_uniform_parts() + size_t(MaxNbParts + 1) * sizeof(int); }
+    void keep_regions() {
+        check(cs_workspace_reserve_scratch(ws_, reference_regions()), "CuSortWorkspace (scratch regions)");
+    }
 
 public:
-    explicit CuSortWorkspace(size_t initialCapacity = 0, int blocksPerSM = 4, int nbSortStreams = 8)
-        : blocks_per_sm_(blocksPerSM), streams_(nbSortStreams) {
+    explicit CuSortWorkspace(size_t initialCapacity = 0, int blocksPerSM = 4, int nbSortStreams = 8) {
         check(cs_workspace_create(&ws_, DType<T>::value, initialCapacity, blocksPerSM < 1 ? 1 : blocksPerSM,
                                   nbSortStreams < 1 ? 1 : nbSortStreams),
               "CuSortWorkspace");
+        keep_regions();
     }
-    ~CuSortWorkspace() { if (ws_) cs_workspace_destroy(ws_); }
+    ~CuSortWorkspace() {
+        if (ws_) cs_workspace_destroy(ws_);
+    }
     CuSortWorkspace(const CuSortWorkspace&) = delete;
     CuSortWorkspace& operator=(const CuSortWorkspace&) = delete;
     CuSortWorkspace(CuSortWorkspace&& o) noexcept
-        : ws_(o.ws_), blocks_per_sm_(o.blocks_per_sm_), streams_(o.streams_) { o.ws_ = nullptr; }
-    void reserve(size_t capacity) { check(cs_workspace_reserve(ws_, capacity), "CuSortWorkspace::reserve"); }
-    void setBlocksPerSM(int blocksPerSM) { blocks_per_sm_ = blocksPerSM < 1 ? 1 : blocksPerSM; }
+        : ws_(o.ws_), host_counters_(std::move(o.host_counters_)),
+          host_uniform_parts_(std::move(o.host_uniform_parts_)),
+          host_large_partitions_(std::move(o.host_large_partitions_)) {
+        o.ws_ = nullptr;
+    }
+
+    void reserve(size_t capacity) {
+        check(cs_workspace_reserve(ws_, capacity), "CuSortWorkspace::reserve");
+        keep_regions();
+    }
+    void setBlocksPerSM(int blocksPerSM) {
+        check(cs_workspace_set_blocks_per_cu(ws_, blocksPerSM < 1 ? 1 : blocksPerSM), "CuSortWorkspace::setBlocksPerSM");
+        keep_regions();
+    }
+
+    T* valuesBuffer() {
+        void* v = nullptr;
+        cs_workspace_info(ws_, &v, nullptr, nullptr, nullptr, nullptr);
+        return static_cast<T*>(v);
+    }
+    T* pivots() { return arena_at<T>(0); }
+    int* counters() { return arena_at<int>(off_counters()); }
+    int* skipScatterFlag() { return counters() ? counters() + MaxNbParts + 1 : nullptr; }
+    int* blockOffsets() { return arena_at<int>(off_block_offsets()); }
+    int* blockUniform() { return arena_at<int>(off_block_uniform()); }
+    int* uniformParts() { return arena_at<int>(off_uniform_parts()); }
+    int* activeNbParts() { return arena_at<int>(off_uniform_parts() + size_t(MaxNbParts) * sizeof(int)); }
     size_t capacity() const {
         size_t c = 0;
         cs_workspace_capacity(ws_, &c);
         return c;
     }
-    int nbSortStreams() const { return streams_; }
+    int persistentBlocks() const {
+        int32_t b = 0;
+        cs_workspace_info(ws_, nullptr, nullptr, nullptr, &b, nullptr);
+        return b;
+    }
+    std::vector<int>& hostCounters() { return host_counters_; }
+    std::vector<int>& hostUniformParts() { return host_uniform_parts_; }
+    std::vector<int>& hostLargePartitions() { return host_large_partitions_; }
+    int nbSortStreams() const {
+        int32_t n = 0;
+        cs_workspace_info(ws_, nullptr, nullptr, nullptr, nullptr, &n);
+        return n;
+    }
+    hipStream_t sortStream(int idx) const {
+        hipStream_t st = nullptr;
+        check(cs_workspace_stream(ws_, idx, &st, nullptr), "CuSortWorkspace::sortStream");
+        return st;
+    }
+    hipEvent_t sortDone(int idx) const {
+        cs_hip_event ev = nullptr;
+        check(cs_workspace_stream(ws_, idx, nullptr, &ev), "CuSortWorkspace::sortDone");
+        return ev;
+    }
+    bool workerEventsRecorded() const {
+        int32_t r = 0;
+        cs_workspace_events_recorded(ws_, &r);
+        return r != 0;
+    }
+    // The library records the events itself; kept for source compatibility.
+    void markWorkerEventsRecorded() {}
     cs_workspace* handle() { return ws_; }
 };
 
-template <typename T> struct CuSortTuning {
-    // Elements from which HybridSort takes the partitioned path.
-    static int64_t PartitionThreshold() { return cs_hybrid_threshold(DType<T>::value); }
+// CuSortTuning<T> (reference cusort.hpp:2476-2483): the element count from
+// which HybridSort / HybridSortV3 take the partitioned path, a specialisable
+// compile-time constant.  The default is the crossover measured on MI355X
+// (merge below, partition above: 2^25 for 4-byte elements, 2^24 for 8-byte
+// ones, 2^22 for 16-byte records), where the reference's default (tuned on
+// its own GPUs) never partitions 4-byte keys and partitions 8-byte ones from
+// 2^27.  A caller's specialisation is what HybridSort consults.
+template <typename T>
+struct CuSortTuning {
+    static constexpr int PartitionThreshold = sizeof(T) >= 16 ? (1 << 22) : sizeof(T) >= 8 ? (1 << 24) : (1 << 25);
 };
 
 // ------------------------------------------------------- large arrays ----
@@ -146,17 +240,53 @@ void SortMerge(T* dataToSort, const int64_t nbValuesToSort, const int nbThreadsP
           "SortMerge");
 }
 
+// Explicit partitioned path with the workspace's buffers (no allocation).
+template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 1024, bool TestIfSorted = false,
+          bool FullSM = true>
+void HybridSortPartitionedV3(T* dataToSort, const int64_t nbValuesToSort, CuSortWorkspace<T, MaxNbParts>& workspace,
+                             hipStream_t stream = nullptr) {
+    static_assert(((MaxValuesToSort - 1) & MaxValuesToSort) == 0, "MaxValuesToSort must be a power of 2");
+    if (nbValuesToSort <= 1) return;
+    workspace.reserve(size_t(nbValuesToSort));
+    check(cs_sort_partition_ws(DType<T>::value, dataToSort, nbValuesToSort, 0, MaxNbParts,
+                               flags<TestIfSorted, FullSM, false>(), workspace.handle(), stream),
+          "HybridSortPartitionedV3");
+}
+
+// Dispatch as the reference's HybridSortV3 (cusort.hpp:2656-2703): below
+// CuSortTuning<T>::PartitionThreshold the merge sort (single tiles need no
+// workspace), above it the partitioned path; the workspace grows to n.
 template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 1024, bool TestIfSorted = false,
           bool FullSM = true>
 void HybridSortV3(T* dataToSort, const int64_t nbValuesToSort, CuSortWorkspace<T, MaxNbParts>& workspace,
                   hipStream_t stream = nullptr) {
-    check(cs_sort_hybrid(DType<T>::value, dataToSort, nbValuesToSort, workspace.handle(), stream), "HybridSortV3");
+    if (nbValuesToSort <= 1) return;
+    if (nbValuesToSort >= CuSortTuning<T>::PartitionThreshold) {
+        HybridSortPartitionedV3<T, MaxValuesToSort, MaxNbParts, TestIfSorted, FullSM>(dataToSort, nbValuesToSort,
+                                                                                     workspace, stream);
+        return;
+    }
+    if (nbValuesToSort <= cs_batched_max_n(DType<T>::value)) {
+        SortMerge<T, MaxValuesToSort, TestIfSorted, FullSM>(dataToSort, nbValuesToSort, 0, stream);
+        return;
+    }
+    workspace.reserve(size_t(nbValuesToSort));
+    check(cs_sort_merge_ws(DType<T>::value, dataToSort, nbValuesToSort, workspace.handle(), stream), "HybridSortV3");
 }
 
+// Without a workspace (reference cusort.hpp:2706-2725): the partitioned path
+// builds a temporary workspace and synchronises the stream before it goes.
 template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 1024, bool TestIfSorted = false,
           bool FullSM = true>
 void HybridSortV3(T* dataToSort, const int64_t nbValuesToSort, hipStream_t stream = nullptr) {
-    check(cs_sort_hybrid(DType<T>::value, dataToSort, nbValuesToSort, nullptr, stream), "HybridSortV3");
+    if (nbValuesToSort < CuSortTuning<T>::PartitionThreshold) {
+        SortMerge<T, MaxValuesToSort, TestIfSorted, FullSM>(dataToSort, nbValuesToSort, 0, stream);
+        return;
+    }
+    CuSortWorkspace<T, MaxNbParts> workspace{static_cast<size_t>(nbValuesToSort)};
+    HybridSortPartitionedV3<T, MaxValuesToSort, MaxNbParts, TestIfSorted, FullSM>(dataToSort, nbValuesToSort,
+                                                                                 workspace, stream);
+    check(int(hipStreamSynchronize(stream)), "HybridSortV3");
 }
 
 template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 1024, bool TestIfSorted = false,
@@ -171,15 +301,6 @@ template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 1024, bool Te
           bool FullSM = true>
 void HybridSort(T* dataToSort, const int64_t nbValuesToSort, hipStream_t stream = nullptr) {
     HybridSortV3<T, MaxValuesToSort, MaxNbParts, TestIfSorted, FullSM>(dataToSort, nbValuesToSort, stream);
-}
-
-template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 1024, bool TestIfSorted = false,
-          bool FullSM = true>
-void HybridSortPartitionedV3(T* dataToSort, const int64_t nbValuesToSort, CuSortWorkspace<T, MaxNbParts>&,
-                             hipStream_t stream = nullptr) {
-    check(cs_sort_partition(DType<T>::value, dataToSort, nbValuesToSort, 32, MaxNbParts,
-                            flags<TestIfSorted, FullSM, false>(), stream),
-          "HybridSortPartitionedV3");
 }
 
 template <typename T, int MaxValuesToSort = 2048, int MaxNbParts = 512, bool TestIfSorted = false,

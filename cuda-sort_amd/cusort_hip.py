@@ -103,6 +103,12 @@ def lib():
         L.cs_workspace_capacity.argtypes = [vp, ctypes.POINTER(sz)]
         L.cs_workspace_destroy.restype = ctypes.c_int
         L.cs_workspace_destroy.argtypes = [vp]
+        L.cs_workspace_calls.restype = ctypes.c_int
+        L.cs_workspace_calls.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.cs_memory_stats.restype = ctypes.c_int
+        L.cs_memory_stats.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.cs_sort_partition_ws.restype = ctypes.c_int
+        L.cs_sort_partition_ws.argtypes = [ctypes.c_int, vp, ctypes.c_int64, i32, i32, ctypes.c_uint32, vp, vp]
         L.cs_sort_merge.restype = ctypes.c_int
         L.cs_sort_merge.argtypes = [ctypes.c_int, vp, i64, i32, i32, vp, sz, ctypes.c_int, vp]
         L.cs_sort_merge_ws.restype = ctypes.c_int
@@ -295,10 +301,29 @@ class Workspace:
         check(lib().cs_workspace_capacity(self.handle, ctypes.byref(c)), "cs_workspace_capacity")
         return int(c.value)
 
+    def calls(self) -> tuple:
+        """(merge sorts, partition sorts) run through this workspace."""
+        m, p = ctypes.c_int64(0), ctypes.c_int64(0)
+        check(lib().cs_workspace_calls(self.handle, ctypes.byref(m), ctypes.byref(p)), "cs_workspace_calls")
+        return int(m.value), int(p.value)
+
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.cs_workspace_destroy(self.handle)
             self.handle = None
+
+
+def memory_stats() -> tuple:
+    """(allocations, bytes) the library has taken from its device pool."""
+    a, b = ctypes.c_int64(0), ctypes.c_int64(0)
+    check(lib().cs_memory_stats(ctypes.byref(a), ctypes.byref(b)), "cs_memory_stats")
+    return int(a.value), int(b.value)
+
+
+def sort_partition_ws(dtype: int, data: int, n: int, ws: "Workspace", min_parts: int = 0, max_parts: int = 0,
+                      flags: int = 0, stream: int = 0) -> None:
+    check(lib().cs_sort_partition_ws(dtype, data, n, min_parts, max_parts, flags, ws.handle, stream),
+          "cs_sort_partition_ws")
 
 
 def sort_merge(dtype: int, data: int, n: int, tile: int = 0, merge_threads: int = 0, ext: int = 0,

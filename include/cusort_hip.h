@@ -201,22 +201,62 @@ int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant);
  * MinNbParts, MaxNbParts> (cusort.hpp:2020-2023, 1869-2017): sample-sort
  * partition into up to 2*max_parts-1 buckets (equality buckets for repeated
  * splitters), one ragged sort of all buckets, merge passes for buckets longer
- * than a chunk.  It synchronises the stream twice (bucket sizes, release of
- * its temporaries); the reference synchronises per recursion level.
+ * than a chunk.  It synchronises the stream once or twice (first-level bucket
+ * starts of a two-level partition, the final plan's 12-byte header) and once
+ * more when a bucket is longer than a chunk; the reference synchronises per
+ * recursion level (cusort.hpp:1977).  Without a workspace its temporaries are
+ * stream-ordered allocations from a per-device pool that keeps freed memory
+ * cached until the process ends.
  *
  * cs_sort_hybrid replaces HybridSort / HybridSortV3 (cusort.hpp:2656-2754):
- * merge below cs_hybrid_threshold(dtype) elements, partition above.  With a
- * workspace of capacity >= n no memory is allocated.                        */
+ * merge below cs_hybrid_threshold(dtype) elements, partition above.
+ *
+ * Workspace (the reference's CuSortWorkspace, cusort.hpp:67-219): a values
+ * buffer of `capacity` elements plus an arena holding every other temporary
+ * of a merge or partition sort of up to `capacity` elements, and a pool of
+ * n_streams worker streams with completion events.  The _ws entry points
+ * and cs_sort_hybrid with a workspace of capacity >= n allocate no device
+ * memory (cs_memory_stats counts the library's allocations); they fail with
+ * CS_E_WORKSPACE when the workspace is too small or of another dtype.  Calls
+ * that share a workspace are ordered through an event (each waits for the
+ * previous one on its own stream); reserve and destroy free and allocate in
+ * stream order behind them, without synchronising the host or the device.
+ * One workspace per host thread (not thread-safe), as in the reference.     */
 typedef struct cs_workspace cs_workspace;
+typedef struct ihipEvent_t* cs_hip_event;
 int cs_workspace_create(cs_workspace** ws, int dtype, size_t capacity, int32_t blocks_per_cu, int32_t n_streams);
 int cs_workspace_reserve(cs_workspace* ws, size_t capacity);
 int cs_workspace_capacity(const cs_workspace* ws, size_t* capacity);
+/* Grows the arena to at least arena_bytes (stream ordered, like reserve). */
+int cs_workspace_reserve_scratch(cs_workspace* ws, size_t arena_bytes);
+/* setBlocksPerSM (cusort.hpp:170-188): bounds the partition path's persistent
+ * grids to blocks_per_cu workgroups per CU. */
+int cs_workspace_set_blocks_per_cu(cs_workspace* ws, int32_t blocks_per_cu);
+/* valuesBuffer(), persistentBlocks(), nbSortStreams() (cusort.hpp:201-218)
+ * and the arena; any pointer may be NULL. */
+int cs_workspace_info(const cs_workspace* ws, void** values, void** arena, size_t* arena_bytes,
+                      int32_t* persistent_blocks, int32_t* n_streams);
+/* sortStream(i), sortDone(i) (cusort.hpp:214-215).  done[0] is recorded on
+ * the caller's stream at the end of every sort that uses the workspace; the
+ * other events are never recorded (waiting on them is a no-op). */
+int cs_workspace_stream(const cs_workspace* ws, int32_t index, hipStream_t* stream, cs_hip_event* done);
+/* workerEventsRecorded() (cusort.hpp:216). */
+int cs_workspace_events_recorded(const cs_workspace* ws, int32_t* recorded);
+/* Sorts run through the workspace by path (merge, partition). */
+int cs_workspace_calls(const cs_workspace* ws, int64_t* merge_calls, int64_t* partition_calls);
 int cs_workspace_destroy(cs_workspace* ws);
+/* Device allocations the library has made from its pool so far (count and
+ * bytes, process wide). */
+int cs_memory_stats(int64_t* pool_allocations, int64_t* pool_bytes);
 int cs_sort_merge(int dtype, void* data, int64_t n, int32_t tile, int32_t merge_threads, void* ext_buf,
                   size_t ext_elems, int stored_in_right_place, hipStream_t stream);
 int cs_sort_merge_ws(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
 int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32_t max_parts, uint32_t flags,
                       hipStream_t stream);
+/* HybridSortPartitionedV3 (cusort.hpp:2488-2653): the partition path with
+ * the workspace's buffers. */
+int cs_sort_partition_ws(int dtype, void* data, int64_t n, int32_t min_parts, int32_t max_parts, uint32_t flags,
+                         cs_workspace* ws, hipStream_t stream);
 int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
 int64_t cs_hybrid_threshold(int dtype);
 

@@ -13,6 +13,17 @@
 
 using cusort_hip::IntAndData;
 
+// A caller's own crossover, as the reference invites (cusort.hpp:2476-2478):
+// HybridSort must consult it.
+template <>
+struct cusort_hip::CuSortTuning<unsigned int> {
+    static constexpr int PartitionThreshold = 1 << 16;
+};
+// The default trait is a compile-time constant, usable as the reference's
+// dispatcher uses it (cusort.hpp:2667-2668).
+static_assert(cusort_hip::CuSortTuning<int>::PartitionThreshold > (1 << 20), "int threshold is a constant");
+static_assert(cusort_hip::CuSortTuning<unsigned int>::PartitionThreshold == (1 << 16), "specialised");
+
 static int failures = 0;
 
 template <class T> bool key_less(const T& a, const T& b) { return a < b; }
@@ -105,6 +116,53 @@ int main() {
         cusort_hip::HybridSortPartitionedV3<unsigned long long>(d, n, ws);
         expect_sorted_equal(h, download(d, h.size()), n, "HybridSortPartitionedV3 u64");
         (void)hipFree(d);
+    }
+    {   // the tuning trait decides the path; the workspace path allocates nothing
+        const int n = 1 << 18;
+        std::vector<unsigned int> h(static_cast<size_t>(n));
+        for (auto& v : h) v = rng();
+        unsigned int* d = upload(h);
+        cusort_hip::CuSortWorkspace<unsigned int> ws{static_cast<size_t>(n)};
+        int64_t merges0 = 0, parts0 = 0, merges1 = 0, parts1 = 0, allocs0 = 0, allocs1 = 0;
+        cs_workspace_calls(ws.handle(), &merges0, &parts0);
+        cs_memory_stats(&allocs0, nullptr);
+        if (n >= cusort_hip::CuSortTuning<unsigned int>::PartitionThreshold)   // reference-style use of the trait
+            cusort_hip::HybridSort<unsigned int>(d, n, ws);
+        cs_workspace_calls(ws.handle(), &merges1, &parts1);
+        cs_memory_stats(&allocs1, nullptr);
+        expect_sorted_equal(h, download(d, h.size()), n, "HybridSort with a specialised CuSortTuning");
+        const bool partitioned = parts1 == parts0 + 1 && merges1 == merges0;
+        std::printf("%s specialised threshold takes the partitioned path\n", partitioned ? "PASS" : "FAIL");
+        std::printf("%s workspace-backed partition sort allocates nothing (%lld allocations)\n",
+                    allocs1 == allocs0 ? "PASS" : "FAIL", (long long)(allocs1 - allocs0));
+        if (!partitioned || allocs1 != allocs0) ++failures;
+        // the same n as int stays below int's threshold: merge path, no allocation either
+        std::vector<int> hi(static_cast<size_t>(n));
+        for (auto& v : hi) v = int(rng());
+        int* di = upload(hi);
+        cusort_hip::CuSortWorkspace<int> wsi{static_cast<size_t>(n)};
+        cs_memory_stats(&allocs0, nullptr);
+        cusort_hip::HybridSortV3<int>(di, n, wsi);
+        cs_memory_stats(&allocs1, nullptr);
+        expect_sorted_equal(hi, download(di, hi.size()), n, "HybridSortV3 int (workspace, merge path)");
+        int64_t m = 0, p = 0;
+        cs_workspace_calls(wsi.handle(), &m, &p);
+        const bool merged = m == 1 && p == 0 && allocs1 == allocs0;
+        std::printf("%s default threshold keeps the merge path without allocating\n", merged ? "PASS" : "FAIL");
+        if (!merged) ++failures;
+        // reference accessors
+        bool acc = ws.valuesBuffer() != nullptr && ws.capacity() >= size_t(n) && ws.nbSortStreams() == 8 &&
+                   ws.persistentBlocks() > 0 && ws.pivots() != nullptr && ws.counters() != nullptr &&
+                   ws.blockOffsets() != nullptr && ws.workerEventsRecorded();
+        hipStream_t worker = ws.sortStream(3);
+        acc = acc && worker != nullptr && hipStreamWaitEvent(worker, ws.sortDone(0), 0) == hipSuccess &&
+              hipStreamSynchronize(worker) == hipSuccess;
+        ws.setBlocksPerSM(2);
+        acc = acc && ws.persistentBlocks() > 0 && ws.blockOffsets() != nullptr;
+        std::printf("%s workspace accessors\n", acc ? "PASS" : "FAIL");
+        if (!acc) ++failures;
+        (void)hipFree(d);
+        (void)hipFree(di);
     }
     {   // key / payload pairs in two arrays: each pair must survive, keys in order
         const int n = 1000, arrays = 40;

@@ -21,8 +21,14 @@ def run_large(cs, orc, rt, dtype, dist, n, method, seed=42, **kw):
         cs.sort_merge_ws(dtype, buf.ptr, n, ws)
     elif method == "partition":
         cs.sort_partition(dtype, buf.ptr, n, **kw)
+    elif method == "partition_ws":
+        ws = cs.Workspace(dtype, n)
+        cs.sort_partition_ws(dtype, buf.ptr, n, ws, **kw)
     elif method == "hybrid":
         cs.sort_hybrid(dtype, buf.ptr, n, kw.get("ws"))
+    elif method == "hybrid_ws":
+        ws = cs.Workspace(dtype, n)
+        cs.sort_hybrid(dtype, buf.ptr, n, ws)
     rt.synchronize()
     got = cs.to_numpy(buf, dtype, n)
     where = f"dtype={dtype} dist={orc.DIST_NAMES[dist]} n={n} method={method}"
@@ -39,7 +45,8 @@ DISTS = ["D_UNIFORM", "D_SORTED", "D_REVERSED", "D_CONSTANT", "D_DUP50", "D_EXTR
          "D_PRESORTED_RUNS"]
 
 
-@pytest.mark.parametrize("method", ["merge", "merge_out", "merge_ws", "partition", "hybrid"])
+@pytest.mark.parametrize("method", ["merge", "merge_out", "merge_ws", "partition", "partition_ws", "hybrid",
+                                    "hybrid_ws"])
 @pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32", "F64"])
 def test_large_methods(cs, orc, gpu, dtype, method):
     dt = getattr(cs, dtype)
@@ -126,3 +133,41 @@ def test_repeated_sorts_keep_memory_bounded(cs, orc, gpu):
     assert free0 - free1 < (64 << 20), (free0, free1)
     got = cs.to_numpy(buf, cs.I32, n)
     assert got.tobytes() == np.sort(host).tobytes()
+
+
+@pytest.mark.parametrize("dtype,n", [("U64", (1 << 24) + 3), ("I32", (1 << 25) + 5), ("I32", (1 << 22) + 9),
+                                     ("REC_I64_I32", (1 << 22) + 1)])
+def test_workspace_sorts_allocate_nothing(cs, orc, gpu, dtype, n):
+    """The reference's allocation-free path (CuSortWorkspace, cusort.hpp:67):
+    with a reserved workspace the hybrid sort, above the threshold (the
+    partition path) and below it (the merge path), takes no device memory
+    from the library's pool, call after call, and still sorts correctly."""
+    dt = getattr(cs, dtype)
+    host = orc.generate(dt, orc.D_UNIFORM, n, n, 11)
+    buf = cs.from_numpy(host)
+    ws = cs.Workspace(dt, n)
+    gpu.synchronize()
+    before = cs.memory_stats()
+    for _ in range(3):
+        buf.upload(host)
+        cs.sort_hybrid(dt, buf.ptr, n, ws)
+    gpu.synchronize()
+    assert cs.memory_stats() == before
+    merges, parts = ws.calls()
+    assert (merges, parts) == ((0, 3) if n >= cs.hybrid_threshold(dt) else (3, 0))
+    got = cs.to_numpy(buf, dt, n)
+    if dt == cs.REC_I64_I32:
+        rc, msg = orc.validate_segmented(dt, host, got, np.array([0, n], dtype=np.int64))
+        assert rc == 0, msg
+    else:
+        assert got.tobytes() == orc.sort_large(dt, host).tobytes()
+
+
+def test_workspace_too_small_is_refused(cs, gpu):
+    n = 1 << 20
+    buf = cs.device_buffer(cs.I32, n)
+    ws = cs.Workspace(cs.I32, n // 2)
+    with pytest.raises(cs.CsError):
+        cs.sort_partition_ws(cs.I32, buf.ptr, n, ws)
+    ws.reserve(n)
+    cs.sort_partition_ws(cs.I32, buf.ptr, n, ws, max_parts=2048)
