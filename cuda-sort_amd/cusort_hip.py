@@ -107,6 +107,10 @@ def lib():
         L.cs_workspace_calls.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.cs_memory_stats.restype = ctypes.c_int
         L.cs_memory_stats.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.cs_sort_merge_static.restype = ctypes.c_int
+        L.cs_sort_merge_static.argtypes = [ctypes.c_int, vp, ctypes.c_int64, vp]
+        L.cs_sort_merge_static_max_bytes.restype = ctypes.c_size_t
+        L.cs_sort_merge_static_max_bytes.argtypes = []
         L.cs_sort_partition_ws.restype = ctypes.c_int
         L.cs_sort_partition_ws.argtypes = [ctypes.c_int, vp, ctypes.c_int64, i32, i32, ctypes.c_uint32, vp, vp]
         L.cs_sort_merge.restype = ctypes.c_int
@@ -318,6 +322,15 @@ def memory_stats() -> tuple:
     a, b = ctypes.c_int64(0), ctypes.c_int64(0)
     check(lib().cs_memory_stats(ctypes.byref(a), ctypes.byref(b)), "cs_memory_stats")
     return int(a.value), int(b.value)
+
+
+def sort_merge_static(dtype: int, data: int, n: int, stream: int = 0) -> None:
+    """cs_sort_merge_static: the merge sort with the library's fixed scratch."""
+    check(lib().cs_sort_merge_static(dtype, data, n, stream), "cs_sort_merge_static")
+
+
+def sort_merge_static_status(dtype: int, data: int, n: int, stream: int = 0) -> int:
+    return int(lib().cs_sort_merge_static(dtype, data, n, stream))
 
 
 def sort_partition_ws(dtype: int, data: int, n: int, ws: "Workspace", min_parts: int = 0, max_parts: int = 0,

@@ -258,6 +258,17 @@ int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32
 int cs_sort_partition_ws(int dtype, void* data, int64_t n, int32_t min_parts, int32_t max_parts, uint32_t flags,
                          cs_workspace* ws, hipStream_t stream);
 int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * it).  The in-place merge sort of cs_sort_merge, with its scratch (n
+ * elements plus one int64 per merge tile) taken from a fixed per-device
+ * array of cs_sort_merge_static_max_bytes() bytes (64 MiB: up to ~16M
+ * 4-byte or ~8M 8-byte elements) that the library holds until the process
+ * ends; larger n fails with CS_E_CAPACITY.  It allocates nothing.  Every call
+ * on one device shares that device's array, so calls that overlap on one
+ * device race; none of the array needs to start at zero. */
+int cs_sort_merge_static(int dtype, void* data, int64_t n, hipStream_t stream);
+size_t cs_sort_merge_static_max_bytes(void);
 int64_t cs_hybrid_threshold(int dtype);
 
 /* ----------------------------------------------------------------- serving
