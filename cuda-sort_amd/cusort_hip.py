@@ -142,6 +142,10 @@ def lib():
         L.cs_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.cs_server_destroy.restype = ctypes.c_int
         L.cs_server_destroy.argtypes = [vp]
+        L.cs_server_inject_launch_failure.restype = ctypes.c_int
+        L.cs_server_inject_launch_failure.argtypes = [vp, i32]
+        L.cs_server_pool_in_use.restype = ctypes.c_int
+        L.cs_server_pool_in_use.argtypes = [vp, P64]
         L.cs_device_copy.restype = ctypes.c_int
         L.cs_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.cs_generate.restype = ctypes.c_int
@@ -214,7 +218,7 @@ class ServerStats(ctypes.Structure):
                 ("keys", ctypes.c_int64), ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double),
                 ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double), ("keys_per_s", ctypes.c_double),
                 ("mean_batch_requests", ctypes.c_double), ("peak_pool_bytes", ctypes.c_int64),
-                ("max_inflight_batches", ctypes.c_int64)]
+                ("max_inflight_batches", ctypes.c_int64), ("bypasses", ctypes.c_int64)]
 
 
 class Server:
@@ -268,6 +272,19 @@ class Server:
 
     def drain(self) -> None:
         check(lib().cs_server_drain(self.handle), "cs_server_drain")
+
+    def done_time(self, ticket: int) -> float:
+        a, d = ctypes.c_double(0), ctypes.c_double(0)
+        check(lib().cs_server_latency(self.handle, ticket, ctypes.byref(a), ctypes.byref(d)), "cs_server_latency")
+        return d.value
+
+    def inject_launch_failure(self, status: int) -> None:
+        check(lib().cs_server_inject_launch_failure(self.handle, status), "cs_server_inject_launch_failure")
+
+    def pool_in_use(self) -> int:
+        b = ctypes.c_int64(0)
+        check(lib().cs_server_pool_in_use(self.handle, ctypes.byref(b)), "cs_server_pool_in_use")
+        return int(b.value)
 
     def stats(self) -> dict:
         st = ServerStats()

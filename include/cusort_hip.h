@@ -276,17 +276,22 @@ int64_t cs_hybrid_threshold(int dtype);
  * counterpart.  A request is a host array of n elements (in) and a host
  * output array (out, may equal in); it completes when out holds the sorted
  * array.  Requests are queued in order; whenever one of `slots` in-flight
- * slots is free the scheduler packs pending requests into one ragged batch
- * (up to max_batch_elems elements and max_batch_requests requests; 0 picks
- * 4M and 4096) provided the device pool, sized to the HBM admission budget
- * `budget_bytes`, can hold it (a request longer than one tile also needs
- * merge scratch of its own size); otherwise it waits for a batch to complete.
- * Each batch is copied in, sorted (one persistent launch plus merge passes
- * for long requests) and copied out on its slot's stream, so batches
- * overlap on the device.  Submission fails with CS_E_BUDGET for a request
- * that could never fit.  Latency = arrival (the caller's timestamp from
- * cs_server_now(), or the submission time when arrival_s < 0) to the moment
- * the result is in `out`.  The server keeps every request until released. */
+ * slots is free the scheduler scans the pending requests in order and packs
+ * into one ragged batch every request that keeps the batch within
+ * max_batch_elems elements and max_batch_requests requests (0 picks 4M and
+ * 4096) and within the largest free block of the device pool, sized to the
+ * HBM admission budget `budget_bytes` (a request longer than one tile also
+ * needs merge scratch of its own size).  A request that does not fit is
+ * passed over, at most 16 times: after that nothing queued behind it is
+ * admitted before it.  When nothing fits the scheduler waits for a batch to
+ * complete or a new request to arrive.  Each batch is copied in, sorted (one
+ * persistent launch plus merge passes for long requests) and copied out on
+ * its slot's stream, so batches overlap on the device, and each completes
+ * as soon as its own work finishes.  Submission fails with CS_E_BUDGET for a
+ * request that could never fit.  Latency = arrival (the caller's timestamp
+ * from cs_server_now(), or the submission time when arrival_s < 0) to the
+ * moment the result is in `out`.  The server keeps every request until
+ * released. */
 typedef struct cs_server cs_server;
 typedef struct cs_server_stats {
     int64_t completed;          /* requests completed successfully */
@@ -298,6 +303,7 @@ typedef struct cs_server_stats {
     double mean_batch_requests; /* average requests per batch */
     int64_t peak_pool_bytes;    /* high-water mark of the device pool */
     int64_t max_inflight_batches;
+    int64_t bypasses;           /* times a request was admitted ahead of an earlier one */
 } cs_server_stats;
 double cs_server_now(void);
 int cs_server_create(cs_server** s, int dtype, size_t budget_bytes, int32_t slots, int64_t max_batch_elems,
@@ -314,6 +320,13 @@ int cs_server_release(cs_server* s, uint64_t ticket);
 int cs_server_drain(cs_server* s);
 int cs_server_get_stats(cs_server* s, cs_server_stats* stats);
 int cs_server_destroy(cs_server* s);
+/* Failure injection for tests of the error path: the next batch launched
+ * fails with `status` (nonzero) after its input copy has been queued; its
+ * requests finish with that status once the copy has drained, and later
+ * batches run normally. */
+int cs_server_inject_launch_failure(cs_server* s, int32_t status);
+/* Bytes of the device pool currently held by batches. */
+int cs_server_pool_in_use(cs_server* s, int64_t* bytes);
 
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel (16-byte
  * accesses when both pointers are 16-byte aligned).  The reference harness

@@ -104,3 +104,64 @@ def test_latency_bookkeeping(cs, orc, gpu):
     st = srv.stats()
     assert st["p50_s"] > 0 and st["p99_s"] >= st["p50_s"] and st["keys"] == 5000
     srv.close()
+
+
+def test_small_requests_pass_a_waiting_large_one(cs, orc, gpu):
+    """A large request that cannot fit while another one is in flight does
+    not hold up the small requests queued behind it, and those complete as
+    soon as their own batch finishes, ahead of the earlier large batch."""
+    dt = cs.REC_I64_I32
+    big_n = 1 << 22                               # 64 MiB, 128 MiB with merge scratch
+    srv = cs.Server(dt, 160 << 20, slots=4)       # room for one large request at a time
+    big1 = make_request(orc, dt, big_n, 21)
+    big2 = make_request(orc, dt, big_n, 22)
+    smalls = [make_request(orc, dt, 1000 + 7 * i, 100 + i) for i in range(50)]
+    t1 = srv.submit(big1)
+    t2 = srv.submit(big2)
+    ts = [srv.submit(a) for a in smalls]
+    for t, a in zip(ts, smalls):
+        check_one(orc, dt, a, srv.result(t))
+    check_one(orc, dt, big1, srv.result(t1))
+    check_one(orc, dt, big2, srv.result(t2))
+    last_small = max(srv.done_time(t) for t in ts)
+    assert last_small < srv.done_time(t1) < srv.done_time(t2)
+    st = srv.stats()
+    assert st["bypasses"] >= 1 and st["completed"] == 52
+    assert st["peak_pool_bytes"] <= 160 << 20
+    srv.close()
+
+
+def test_failed_launch_drains_and_recovers(cs, orc, gpu):
+    """A batch whose launch fails after its input copy was queued reports the
+    failure to its requests only; the slot's stream is drained before its
+    pool block and staging buffers are reused, so every later request, on
+    every slot, still sorts correctly and the pool empties."""
+    dt = cs.REC_I64_I32
+    srv = cs.Server(dt, 256 << 20, slots=2)
+    failed = make_request(orc, dt, 1 << 20, 31)
+    srv.inject_launch_failure(cs.E_NO_DEVICE)
+    tf = srv.submit(failed)
+    assert srv.wait(tf) == cs.E_NO_DEVICE
+    srv.drain()
+    assert srv.pool_in_use() == 0
+    rng = np.random.default_rng(5)
+    reqs = [make_request(orc, dt, int(n), 200 + i) for i, n in enumerate(rng.integers(16, 300000, 40))]
+    tickets = []
+    for i, a in enumerate(reqs):
+        if i == 10:
+            srv.inject_launch_failure(cs.E_NO_DEVICE)   # fails one batch among those in flight
+        tickets.append(srv.submit(a))
+    statuses = [srv.wait(t) for t in tickets]
+    assert set(statuses) <= {0, cs.E_NO_DEVICE} and statuses.count(cs.E_NO_DEVICE) >= 1
+    for t, a, rc in zip(tickets, reqs, statuses):
+        if rc == 0:
+            check_one(orc, dt, a, srv.result(t))
+    srv.drain()
+    assert srv.pool_in_use() == 0
+    # the server keeps working on both slots afterwards
+    after = [make_request(orc, dt, int(n), 300 + i) for i, n in enumerate(rng.integers(16, 300000, 20))]
+    tickets = [srv.submit(a) for a in after]
+    for t, a in zip(tickets, after):
+        check_one(orc, dt, a, srv.result(t))
+    assert srv.stats()["completed"] == statuses.count(0) + len(after)
+    srv.close()
