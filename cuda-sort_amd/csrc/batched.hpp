@@ -84,24 +84,24 @@ constexpr bool xlane_shape(int dt, int logw, int loge) {
 }
 
 template <int DT, int LW>
-hipError_t launch_xlane(const void* in, void* out, int64_t arrays, int n, int, unsigned, int, hipStream_t s) {
+hipError_t launch_xlane(const void* in, void* out, int64_t arrays, int n, int, unsigned flags, int, hipStream_t s) {
     using S = typename Elem<DT>::Storage;
     constexpr int apb = 4 * (64 >> (LW - xl::kLogE));
     const int64_t blocks = (arrays + apb - 1) / apb;
     if (blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_bitonic_xlane<Elem<DT>, LW, kXlaneWindow>), dim3(unsigned(blocks)), dim3(256), 0, s,
-                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n);
+                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n, flags);
     return hipGetLastError();
 }
 
 template <int DT, int LW, int LE>
-hipError_t launch_tile(const void* in, void* out, int64_t arrays, int n, int, unsigned, int, hipStream_t s) {
+hipError_t launch_tile(const void* in, void* out, int64_t arrays, int n, int, unsigned flags, int, hipStream_t s) {
     using C = TileCfg<Elem<DT>, LW, LE>;
     using S = typename Elem<DT>::Storage;
     const int64_t blocks = (arrays + C::APB - 1) / C::APB;
     if (blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_bitonic_tile<Elem<DT>, LW, LE>), dim3(unsigned(blocks)), dim3(C::BLOCK), 0, s,
-                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n);
+                       static_cast<const S*>(in), static_cast<S*>(out), arrays, n, flags);
     return hipGetLastError();
 }
 

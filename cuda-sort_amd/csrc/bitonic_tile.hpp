@@ -203,6 +203,68 @@ struct Tile {
     }
 };
 
+// TESTSORTED (the reference's TestIfSorted, cusort_v6.hpp:103-142): is the
+// freshly loaded array already non-decreasing?  The tile is still in the
+// lane-striped load order (slot q*VS + j of lane t = position (q*T + t)*VS
+// + j; positions >= valid hold the maximum key, so a sorted valid prefix
+// stays sorted).  Each lane checks its groups in registers; the first key
+// of every group goes through the group's LDS region so that lane t can
+// compare its group ends with the next positions (for t = T-1 that is
+// lane 0's next group).  Group-uniform result; a tile spanning several
+// waves makes it uniform over the workgroup, whose network barriers need
+// every lane.  A sorted array is a fixed point of the network (no strict
+// inversion), so skipping the network leaves the output unchanged.
+template <class E, int LOGW, int LOGE, int VS>
+__device__ __forceinline__ bool tile_striped_sorted(const typename E::Key* k, int t, typename E::Key* lk) {
+    using C = TileCfg<E, LOGW, LOGE>;
+    using Key = typename E::Key;
+    constexpr int NE = C::NE, T = C::T, NG = NE / VS;
+    bool ok = true;
+    sfor<0, NG>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        sfor<0, VS - 1>([&](auto j_) {
+            constexpr int j = decltype(j_)::value;
+            ok = ok && !(k[q * VS + j + 1] < k[q * VS + j]);
+        });
+    });
+    if constexpr (T == 1) {
+        sfor<0, NG - 1>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            ok = ok && !(k[(q + 1) * VS] < k[q * VS + VS - 1]);
+        });
+        return ok;
+    } else {
+        auto sync = [] {
+            if constexpr (C::kMultiWave) {
+                __syncthreads();
+            } else {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        };
+        constexpr int FLAG = NG * T;   // one word past the group-first keys
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            lk[q * T + t] = k[q * VS];
+        });
+        if (t == 0) lk[FLAG] = Key(1);
+        sync();
+        sfor<0, NG>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            const int x = q * T + t + 1;
+            const Key nx = lk[x < FLAG ? x : 0];
+            ok = ok && (x >= FLAG || !(nx < k[q * VS + VS - 1]));
+        });
+        if (!ok) lk[FLAG] = Key(0);
+        sync();
+        const bool sorted = lk[FLAG] != Key(0);
+        sync();   // the region is reused by the network
+        if constexpr (C::kMultiWave && C::APB > 1) return __syncthreads_and(sorted ? 1 : 0) != 0;
+        return sorted;
+    }
+}
+
 // ------------------------------------------------------------ the kernel --
 // Global I/O is lane-striped (coalesced 16-byte accesses).  Keys without
 // payload are loaded straight into the registers in that order and the
@@ -217,9 +279,11 @@ struct Tile {
 // from src into dst with the T lanes t = 0..T-1 of this group; lk / lp are
 // the group's LDS region (C::REGION words per buffer).  Every lane of the
 // workgroup must call it when C::kMultiWave (it contains barriers).
+// With test (TESTSORTED) an array found sorted right after the load is
+// stored unchanged without running the network.
 template <class E, int LOGW, int LOGE>
 __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typename E::Storage* dst, int valid,
-                                          int t, typename E::Key* lk, int32_t* lp) {
+                                          int t, typename E::Key* lk, int32_t* lp, bool test = false) {
     using C = TileCfg<E, LOGW, LOGE>;
     using Storage = typename E::Storage;
     constexpr int NE = C::NE;
@@ -295,13 +359,15 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
             }
         });
     }
-    if constexpr (E::kPayload && C::NTRANS > 0) tile.template io_transpose<VS, true>(t, lk, lp, kBufFirst);
-
-    // ---- network
-    tile.template run<0>(t, lk, lp);
+    // ---- network (skipped for an array found sorted under TESTSORTED: the
+    // tile is then still in the striped order the stores use)
+    if (!(test && tile_striped_sorted<E, LOGW, LOGE, VS>(tile.k, t, lk))) {
+        if constexpr (E::kPayload && C::NTRANS > 0) tile.template io_transpose<VS, true>(t, lk, lp, kBufFirst);
+        tile.template run<0>(t, lk, lp);
+        if constexpr (C::NTRANS > 0) tile.template io_transpose<VS, false>(t, lk, lp, kBufLast);
+    }
 
     // ---- store, lane-striped
-    if constexpr (C::NTRANS > 0) tile.template io_transpose<VS, false>(t, lk, lp, kBufLast);
     auto store_group = [&](auto q_) {
         constexpr int q = decltype(q_)::value;
         Storage tmp[V];
@@ -342,7 +408,7 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
 // Batched launch: array a = in + a*n, APB arrays per workgroup.
 template <class E, int LOGW, int LOGE>
 __global__ void __launch_bounds__((TileCfg<E, LOGW, LOGE>::BLOCK))
-k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out, int64_t arrays, int n) {
+k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out, int64_t arrays, int n, unsigned flags) {
     using C = TileCfg<E, LOGW, LOGE>;
     __shared__ typename E::Key lds_k[C::LDS_KEYS];
     __shared__ int32_t lds_p[C::LDS_PAYLOAD];
@@ -353,7 +419,8 @@ k_bitonic_tile(const typename E::Storage* in, typename E::Storage* out, int64_t 
     const bool active = a < arrays;
     tile_sort<E, LOGW, LOGE>(in + (active ? a : 0) * int64_t(n), out + (active ? a : 0) * int64_t(n),
                              active ? n : 0, t, lds_k + (C::NTRANS ? group * C::REGION : 0),
-                             lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
+                             lds_p + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0),
+                             (flags & CS_FLAG_TESTSORTED) != 0);
 }
 
 }  // namespace cs

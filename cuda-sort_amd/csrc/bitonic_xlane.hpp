@@ -349,15 +349,69 @@ __device__ __forceinline__ void xl_store(const typename E::Key (&out)[xl::kE], t
     }
 }
 
+// TESTSORTED: is the loaded slot already non-decreasing?  raw is in the
+// striped load order (register 4 q + j of lane t = position 4 (q T + t) +
+// j); positions >= valid count as the padding key.  Lane t compares its
+// 16-byte groups in registers and each group's last key with the next
+// position, whose key it reads from the first keys the lanes leave in the
+// slot's LDS region (for t = T - 1: lane 0's next group).  Slot-uniform
+// result (the lanes of a slot share one wave).
+template <class E, int LOGW>
+__device__ __forceinline__ bool xl_sorted(const typename E::Storage (&raw)[xl::kE], int valid, int t,
+                                          typename E::Key* lk) {
+    using K = typename E::Key;
+    constexpr int T = 1 << (LOGW - xl::kLogE);
+    constexpr int NG = xl::kE / 4;
+    K k[xl::kE];
+    sfor<0, xl::kE>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        k[r] = ((r / 4) * T + t) * 4 + r % 4 < valid ? E::key(raw[r]) : E::kPad;
+    });
+    bool ok = true;
+    sfor<0, xl::kE - 1>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        if constexpr (r % 4 != 3 || T == 1) ok = ok && !(k[r + 1] < k[r]);
+    });
+    if constexpr (T == 1) return ok;
+    constexpr int FLAG = NG * T;   // one word past the group-first keys
+    sfor<0, NG>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        lk[q * T + t] = k[4 * q];
+    });
+    if (t == 0) lk[FLAG] = K(1);
+    wave_lds_fence();
+    sfor<0, NG>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        const int x = q * T + t + 1;
+        const K nx = lk[x < FLAG ? x : 0];
+        ok = ok && (x >= FLAG || !(nx < k[4 * q + 3]));
+    });
+    if (!ok) lk[FLAG] = K(0);
+    wave_lds_fence();
+    const bool sorted = lk[FLAG] != K(0);
+    wave_lds_fence();   // the region is reused by the network
+    return sorted;
+}
+
 // Sorts one array of `valid` keys (valid <= W; 0 for an idle slot) from src
 // into dst with the T lanes of logical index t; lk is the array's LDS region.
+// With test (TESTSORTED) a slot found sorted right after the load is stored
+// unchanged without running the network (a sorted array is a fixed point of
+// every layer, so the output is the same).
 template <class E, int LOGW, int LDSK>
 __device__ __forceinline__ void xlane_sort(const typename E::Storage* src, typename E::Storage* dst, int valid,
-                                           int t, typename E::Key* lk) {
+                                           int t, typename E::Key* lk, bool test = false) {
     typename E::Storage raw[xl::kE];
     typename E::Key out[xl::kE];
     xl_load<E, LOGW>(raw, src, valid, t);
-    xl_network<E, LOGW, LDSK>(raw, out, valid, t, lk);
+    if (test && xl_sorted<E, LOGW>(raw, valid, t, lk)) {
+        sfor<0, xl::kE>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            out[r] = E::key(raw[r]);
+        });
+    } else {
+        xl_network<E, LOGW, LDSK>(raw, out, valid, t, lk);
+    }
     xl_store<E, LOGW>(out, dst, valid, t);
 }
 
@@ -365,7 +419,7 @@ __device__ __forceinline__ void xlane_sort(const typename E::Storage* src, typen
 // workgroups, array a = in + a * n.
 template <class E, int LOGW, int LDSK>
 __global__ void __launch_bounds__(256) k_bitonic_xlane(const typename E::Storage* in, typename E::Storage* out,
-                                                       int64_t arrays, int n) {
+                                                       int64_t arrays, int n, unsigned flags) {
     constexpr int LOGT = LOGW - xl::kLogE;
     constexpr int APW = 64 >> LOGT;
     constexpr int REGION = xl::region_words(LOGW);
@@ -377,7 +431,7 @@ __global__ void __launch_bounds__(256) k_bitonic_xlane(const typename E::Storage
     const int64_t a = int64_t(blockIdx.x) * (4 * APW) + slot;
     const bool active = a < arrays;
     xlane_sort<E, LOGW, LDSK>(in + (active ? a : 0) * int64_t(n), out + (active ? a : 0) * int64_t(n),
-                              active ? n : 0, t, lds + slot * REGION);
+                              active ? n : 0, t, lds + slot * REGION, (flags & CS_FLAG_TESTSORTED) != 0);
 }
 
 }  // namespace cs

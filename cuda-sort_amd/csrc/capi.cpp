@@ -73,7 +73,6 @@ int choose(int dtype, int32_t n, int method, int32_t threads, uint32_t flags, Ch
     const int logref = logn < 5 ? 5 : logn;          // W = max(32, pow2(n))
     const int wref = 1 << logref;
     const bool fullsm = (flags & CS_FLAG_FULLSM) != 0;
-    const bool testsorted = (flags & CS_FLAG_TESTSORTED) != 0;
     const int tmax = wref < 1024 ? wref : 1024;
 
     auto use_lds = [&](int logw, int t) {
@@ -92,14 +91,43 @@ int choose(int dtype, int32_t n, int method, int32_t threads, uint32_t flags, Ch
         return half < 32 ? 32 : (half > 256 ? 256 : half);
     };
 
+    // FULLSM: every exchange through LDS.  The register tile's cross-lane
+    // moves are LDS transposes, but the cross-lane engine that serves
+    // key-only 4-byte arrays by default moves keys between lanes with DPP and
+    // permlane instead: FULLSM takes the 16-per-lane register tile there
+    // (int32), or the LDS network for element types without that tiling.
+    auto use_fullsm_tile = [&](int logw) {
+        const int loge = logw < 4 ? logw : 4;
+        if (xlane_shape(dtype, logw, auto_loge(dtype, logw))) {
+            const KernelInfo k = tile_kernel(dtype, logw, loge);
+            if (k.launch && loge != auto_loge(dtype, logw)) return use_tile(logw, loge);
+            return use_lds(logw < 5 ? 5 : logw, lds_default_threads(logw < 5 ? 5 : logw));
+        }
+        return use_tile(logw, auto_loge(dtype, logw));
+    };
+    // v3 / v4 / v5 with a power-of-two block of t threads over W elements:
+    // the register tile of W / t elements per lane when one exists for the
+    // element type (TESTSORTED is checked inside it), else the LDS network.
+    auto use_block_tile = [&](int logw, int t) {
+        if (is_pow2(t) && t <= (1 << logw)) {
+            const int loge = logw - ceil_log2(t);
+            if (loge >= 1 && loge <= 5) {
+                const KernelInfo k = tile_kernel(dtype, logw, loge);
+                const bool lane_moves = xlane_shape(dtype, logw, loge);   // not LDS-only
+                if (k.launch && !(fullsm && lane_moves)) return use_tile(logw, loge);
+            }
+        }
+        return use_lds(logw, t);
+    };
+
     switch (method) {
         case CS_METHOD_AUTO: {
-            if (fullsm || testsorted) return use_lds(logref, lds_default_threads(logref));
+            if (fullsm) return use_fullsm_tile(logn);
             return use_tile(logn, auto_loge(dtype, logn));
         }
         case CS_BITONIC_WARP: {
             if (n > 32 || (threads != 0 && threads != 32)) return CS_E_UNSUPPORTED;
-            if (fullsm || testsorted) return use_lds(5, 32);
+            if (fullsm) return use_fullsm_tile(logn);
             return use_tile(logn, auto_loge(dtype, logn));
         }
         case CS_BITONIC_V1:
@@ -117,23 +145,22 @@ int choose(int dtype, int32_t n, int method, int32_t threads, uint32_t flags, Ch
             // at every size, test.cu:319-355); v4 keeps bench.cu's <= W.
             const int tcap = method == CS_BITONIC_V3 ? 1024 : tmax;
             if (t % 32 != 0 || t < 32 || t > tcap) return CS_E_UNSUPPORTED;
-            return use_lds(logref, t);
+            return use_block_tile(logref, t);
         }
         case CS_BITONIC_V5_STATIC: {
             if (n < 32) return CS_E_UNSUPPORTED;
             const int t = threads ? threads : (tmax < 256 ? tmax : 256);
             if (!is_pow2(t) || t < 32 || t > tmax) return CS_E_UNSUPPORTED;
-            return use_lds(logref, t);
+            return use_block_tile(logref, t);
         }
         case CS_BITONIC_V6_REGISTER: {
             if (n < 32) return CS_E_UNSUPPORTED;
             int t = threads;
             if (t == 0) t = wref / 16 < 32 ? 32 : wref / 16;
             if (!is_pow2(t) || t < 32 || t > tmax || wref / t > 16) return CS_E_UNSUPPORTED;
-            if (fullsm || testsorted) return use_lds(logref, t);
             const int loge = logref - ceil_log2(t);
             const KernelInfo k = tile_kernel(dtype, logref, loge);
-            if (k.launch) return use_tile(logref, loge);
+            if (k.launch) return use_tile(logref, loge);   // both flags are handled inside the tile
             return use_lds(logref, t);   // dtypes without that tiling: same output
         }
         default: return CS_E_INVALID_VALUE;

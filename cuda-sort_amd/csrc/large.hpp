@@ -236,14 +236,14 @@ hipError_t large_chunk_sort(const void* in, void* out, int64_t n, hipStream_t s)
     const int64_t tail = n - full * chunk;
     if (full > 0) {
         hipLaunchKernelGGL((k_bitonic_tile<E, LW, LE>), dim3(unsigned((full + C::APB - 1) / C::APB)), dim3(C::BLOCK),
-                           0, s, static_cast<const S*>(in), static_cast<S*>(out), full, int(chunk));
+                           0, s, static_cast<const S*>(in), static_cast<S*>(out), full, int(chunk), 0u);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (tail > 0) {
         hipLaunchKernelGGL((k_bitonic_tile<E, LW, LE>), dim3(1), dim3(C::BLOCK), 0, s,
                            static_cast<const S*>(in) + full * chunk, static_cast<S*>(out) + full * chunk, int64_t(1),
-                           int(tail));
+                           int(tail), 0u);
         return hipGetLastError();
     }
     return hipSuccess;

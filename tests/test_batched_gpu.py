@@ -106,6 +106,39 @@ def test_every_method_and_thread_count(cs, orc, gpu, dtype, n):
                 run_case(cs, orc, gpu, dt, dist, n, 7, method, threads, flags)
 
 
+@pytest.mark.parametrize("dtype", ["I32", "F32", "REC_I32_I32", "U64", "REC_I64_I32"])
+@pytest.mark.parametrize("n", [32, 100, 1024, 2048, 4096])
+def test_testsorted_mixed_arrays(cs, orc, gpu, dtype, n):
+    """TESTSORTED inside the register engines: an array found sorted right
+    after the load skips the network.  Already-sorted (host std::sort) and
+    unsorted arrays alternate, so one workgroup holds both kinds, with
+    several arrays per wave and, for the long records, several waves per
+    array (the skip must then be taken by the whole workgroup or not at
+    all).  Every method whose domain holds n runs with the flag."""
+    dt = getattr(cs, dtype)
+    if n > cs.batched_max_n(dt):
+        pytest.skip("beyond capacity")
+    arrays = 37
+    raw = orc.generate(dt, orc.D_UNIFORM, n, n * arrays, 5)
+    presorted = orc.sort_batched(dt, raw, n)
+    host = raw.copy()
+    for a in range(0, arrays, 2):
+        host[a * n:(a + 1) * n] = presorted[a * n:(a + 1) * n]
+    if host.dtype.names:   # records: the payload is the record's input position (the validator's convention)
+        host["payload"] = np.arange(host.size, dtype=np.int32)
+    src = cs.from_numpy(host)
+    for method, threads in method_domain(cs, n):
+        for flags in (cs.FLAG_TESTSORTED, cs.FLAG_TESTSORTED | cs.FLAG_FULLSM):
+            out = cs.device_buffer(dt, n * arrays)
+            cs.batched_sort(dt, src.ptr, out.ptr, arrays, n, method, threads, flags)
+            gpu.synchronize()
+            res = cs.to_numpy(out, dt, n * arrays)
+            rc, msg = orc.validate_batched(dt, host, res, n)
+            assert rc == 0, f"dtype={dtype} n={n} method={cs.METHODS[method]} threads={threads} flags={flags}: {msg}"
+            for a in range(0, arrays, 2):   # a sorted array leaves unchanged, records included
+                assert res[a * n:(a + 1) * n].tobytes() == host[a * n:(a + 1) * n].tobytes()
+
+
 @pytest.mark.parametrize("inplace", [False, True])
 def test_config1_full_workload(cs, orc, gpu, inplace):
     """4096 x 1024 int32, uniform full range, seed 42 (the headline config)."""
