@@ -81,11 +81,92 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(int64_t* v, int64_t
     }
 }
 
-__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts, unsigned char* equal) {
+__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts, unsigned char* equal,
+                              int* zero, int nzero) {
+    if (blockIdx.x == 0)
+        for (int i = int(threadIdx.x); i < nzero; i += int(blockDim.x)) zero[i] = 0;
     const PartSeg g = segs[blockIdx.x];
     for (int b = int(threadIdx.x); b < g.nb; b += int(blockDim.x)) {
         starts[g.first + b] = offsets[g.cell0 + int64_t(b) * g.stride];
         equal[g.first + b] = (g.equal || (b & 1)) ? 1 : 0;
+    }
+}
+
+// Second partition level, laid out on the device from the first level's
+// scanned count matrix (bucket b starts at cell (b, 0) = counts1[b *
+// blocks1]; no host round trip): first-level bucket b becomes segment b.  An equality bucket (odd b) or a bucket no longer than `target` is
+// carried through as one bucket; any other is split by the p2 - 1 global
+// splitters inside it (first-level splitter j is global splitter
+// (j + 1) p2 - 1).  Empty buckets get no workgroup and no bucket, so the
+// final buckets stay in array order.  One workgroup; writes the segment
+// table (nb1 entries), the count / scatter work list and
+// hdr = {work items, count-matrix cells, final buckets}.
+constexpr int kL2Threads = 1024;
+
+__global__ void __launch_bounds__(kL2Threads)
+k_part_level2(const int64_t* counts1, int64_t blocks1, int nb1, int64_t n, int p1, int p2, int64_t target,
+              int64_t block_elems, PartBlock* work, PartSeg* segs, int64_t* hdr) {
+    __shared__ int64_t sc[3][kL2Threads];
+    const int tid = int(threadIdx.x);
+    const int per = (nb1 + kL2Threads - 1) / kL2Threads;
+    const int b0 = tid * per < nb1 ? tid * per : nb1;
+    const int b1 = b0 + per < nb1 ? b0 + per : nb1;
+    struct Geo {
+        int64_t begin, end, blocks;
+        int ns, nbs;
+    };
+    auto geo = [&](int b) {
+        Geo g;
+        g.begin = counts1[int64_t(b) * blocks1];
+        g.end = b + 1 < nb1 ? counts1[int64_t(b + 1) * blocks1] : n;
+        const int64_t len = g.end - g.begin;
+        g.ns = ((b & 1) || len <= target) ? 0 : p2 - 1;
+        g.blocks = len > 0 ? (len + block_elems - 1) / block_elems : 0;
+        g.nbs = len > 0 ? 2 * g.ns + 1 : 0;
+        return g;
+    };
+    int64_t nblk = 0, nbkt = 0, ncel = 0;
+    for (int b = b0; b < b1; ++b) {
+        const Geo g = geo(b);
+        nblk += g.blocks;
+        nbkt += g.nbs;
+        ncel += g.blocks * g.nbs;
+    }
+    sc[0][tid] = nblk;
+    sc[1][tid] = nbkt;
+    sc[2][tid] = ncel;
+    __syncthreads();
+    for (int w = 1; w < kL2Threads; w <<= 1) {   // inclusive scans
+        const int64_t a0 = tid >= w ? sc[0][tid - w] : 0;
+        const int64_t a1 = tid >= w ? sc[1][tid - w] : 0;
+        const int64_t a2 = tid >= w ? sc[2][tid - w] : 0;
+        __syncthreads();
+        sc[0][tid] += a0;
+        sc[1][tid] += a1;
+        sc[2][tid] += a2;
+        __syncthreads();
+    }
+    int64_t blk = sc[0][tid] - nblk, bkt = sc[1][tid] - nbkt, cel = sc[2][tid] - ncel;
+    for (int b = b0; b < b1; ++b) {
+        const Geo g = geo(b);
+        const int j = b / 2;
+        const bool split = g.ns > 0;
+        segs[b] = PartSeg{cel, bkt, int32_t(g.blocks), g.nbs, (b & 1) ? 1 : 0, 0};
+        for (int64_t k = 0; k < g.blocks; ++k) {
+            const int64_t x0 = g.begin + k * block_elems;
+            work[blk + k] = PartBlock{x0, x0 + block_elems < g.end ? x0 + block_elems : g.end, cel + k,
+                                      int32_t(g.blocks), split ? j * p2 : 0, g.ns,
+                                      split && j > 0 ? j * p2 - 1 : -1, split && j < p1 - 1 ? (j + 1) * p2 - 1 : -1,
+                                      0};
+        }
+        blk += g.blocks;
+        bkt += g.nbs;
+        cel += g.blocks * g.nbs;
+    }
+    if (tid == kL2Threads - 1) {
+        hdr[0] = blk;
+        hdr[1] = cel;
+        hdr[2] = bkt;
     }
 }
 
@@ -109,18 +190,18 @@ struct PlanTable {
     int variant[kPlanClasses];
 };
 
-// meta: ccount[16], cursor[16], cstart[16], istart[16], hdr[4]
+// meta: ccount[16], cursor[16], (unused)[32], hdr[4], then the persistent
+// kernels' two item counters
 constexpr int kPlanMeta = 4 * kPlanClasses + 4;
 
-__global__ void k_plan_zero(int* meta) {
-    if (threadIdx.x < kPlanMeta) meta[threadIdx.x] = 0;
-}
 
-__global__ void k_plan_classify(int64_t* starts, const unsigned char* equal, int skip_equal, int64_t nb, int64_t n,
-                                int64_t chunk, unsigned char* cls, int* ccount) {
+__global__ void k_plan_classify(int64_t* starts, const unsigned char* equal, int skip_equal, int64_t nb,
+                                const int64_t* d_nb, int64_t n, int64_t chunk, unsigned char* cls, int* ccount,
+                                int* long_list) {
     __shared__ int h[kPlanClasses];
     if (threadIdx.x < kPlanClasses) h[threadIdx.x] = 0;
     __syncthreads();
+    if (d_nb) nb = *d_nb;   // bucket count of a level planned on the device
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < nb) {
         const int64_t len = (i + 1 < nb ? starts[i + 1] : n) - starts[i];
@@ -129,38 +210,47 @@ __global__ void k_plan_classify(int64_t* starts, const unsigned char* equal, int
         else if (len > chunk) c = kPlanLong;
         else if (len > 0) c = len > 1 ? 64 - __clzll(uint64_t(len - 1)) : 0;
         cls[i] = static_cast<unsigned char>(c);
-        if (c < kPlanClasses) atomicAdd(&h[c], 1);
+        if (c < kPlanLong) atomicAdd(&h[c], 1);
+        if (c == kPlanLong) long_list[atomicAdd(&ccount[kPlanLong], 1)] = int(i);   // rare
     }
     if (i == 0) starts[nb] = n;   // closing offset of the segment CSR
     __syncthreads();
-    if (threadIdx.x < kPlanClasses && h[threadIdx.x]) atomicAdd(&ccount[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x < kPlanLong && h[threadIdx.x]) atomicAdd(&ccount[threadIdx.x], h[threadIdx.x]);
 }
 
-__global__ void k_plan_layout(int* meta, PlanTable t) {
-    if (threadIdx.x != 0) return;
-    const int* ccount = meta;
-    int* cstart = meta + 2 * kPlanClasses;
-    int* istart = meta + 3 * kPlanClasses;
-    int* hdr = meta + 4 * kPlanClasses;
+// Layout of the class-grouped list and the item ranges, classes from maxb
+// down: cstart[c] = first list entry of class c, istart[c] = its first
+// work item.  Every workgroup of k_plan_fill / k_plan_items recomputes it
+// (sixteen classes) from the class counts instead of a launch of its own.
+struct PlanLayout {
+    int cstart[kPlanClasses];
+    int istart[kPlanClasses];
+    int split;   // work items of the multi-wave kernel
+    int items;   // all work items
+};
+
+__device__ void plan_layout(const int* ccount, const PlanTable& t, PlanLayout* L) {
     int seg = 0, items = 0, split = 0;
     for (int c = t.maxb; c >= 0; --c) {
         const int k = (ccount[c] + t.spi[c] - 1) / t.spi[c];
-        cstart[c] = seg;
-        istart[c] = items;
+        L->cstart[c] = seg;
+        L->istart[c] = items;
         seg += ccount[c];
         items += k;
         if (c > t.small_max) split += k;
     }
-    hdr[0] = split;
-    hdr[1] = items;
-    hdr[2] = ccount[kPlanLong];
+    L->split = split;
+    L->items = items;
 }
 
-__global__ void k_plan_fill(const unsigned char* cls, int64_t nb, int* meta, int* list) {
+__global__ void k_plan_fill(const unsigned char* cls, int64_t nb, const int64_t* d_nb, int* meta, PlanTable t,
+                            int* list) {
     __shared__ int h[kPlanClasses], base[kPlanClasses];
+    __shared__ PlanLayout L;
     int* cursor = meta + kPlanClasses;
-    const int* cstart = meta + 2 * kPlanClasses;
     if (threadIdx.x < kPlanClasses) h[threadIdx.x] = 0;
+    if (threadIdx.x == 0) plan_layout(meta, t, &L);
+    if (d_nb) nb = *d_nb;
     __syncthreads();
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int c = i < nb ? int(cls[i]) : 255;
@@ -168,23 +258,30 @@ __global__ void k_plan_fill(const unsigned char* cls, int64_t nb, int* meta, int
     __syncthreads();
     if (threadIdx.x < kPlanClasses && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
     __syncthreads();
-    if (c < kPlanLong) list[cstart[c] + base[c] + local] = int(i);
+    if (c < kPlanLong) list[L.cstart[c] + base[c] + local] = int(i);
 }
 
-__global__ void k_plan_items(const int* meta, PlanTable t, SegItem* items) {
+// Work items of the class-grouped list; workgroup 0 also writes the item
+// ranges the persistent kernels read: hdr = {multi-wave items, all items}.
+__global__ void k_plan_items(int* meta, PlanTable t, SegItem* items) {
+    __shared__ PlanLayout L;
     const int* ccount = meta;
-    const int* cstart = meta + 2 * kPlanClasses;
-    const int* istart = meta + 3 * kPlanClasses;
-    const int* hdr = meta + 4 * kPlanClasses;
+    int* hdr = meta + 4 * kPlanClasses;
+    if (threadIdx.x == 0) plan_layout(ccount, t, &L);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        hdr[0] = L.split;
+        hdr[1] = L.items;
+    }
     const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= hdr[1]) return;
+    if (k >= L.items) return;
     int c = t.maxb;   // classes run from maxb down, their item ranges upward
-    while (c > 0 && k >= istart[c] + (ccount[c] + t.spi[c] - 1) / t.spi[c]) --c;
-    const int j = k - istart[c];
+    while (c > 0 && k >= L.istart[c] + (ccount[c] + t.spi[c] - 1) / t.spi[c]) --c;
+    const int j = k - L.istart[c];
     SegItem it;
     it.bucket = c;
     it.variant = t.variant[c];
-    it.first = cstart[c] + j * t.spi[c];
+    it.first = L.cstart[c] + j * t.spi[c];
     it.count = min(t.spi[c], ccount[c] - j * t.spi[c]);
     items[k] = it;
 }
@@ -228,94 +325,6 @@ namespace {
 
 constexpr int64_t kPartBlockElems = 65536;   // elements per workgroup of count / scatter
 constexpr int kPartOneLevel = 256;           // parts up to which one level is used
-
-// One partition level: the workgroups of every segment, the segments' bucket
-// start descriptors, and per segment its end and whether its buckets are
-// equality buckets by construction (a first-level equality bucket).
-struct Level {
-    std::vector<PartBlock> work;
-    std::vector<PartSeg> segs;
-    std::vector<int64_t> seg_end;
-    std::vector<char> seg_equal;
-    int64_t cells = 0;
-    int64_t nbuckets = 0;
-    int max_ns = 0;
-};
-
-void clear_level(Level& L) {
-    L.work.clear();
-    L.segs.clear();
-    L.seg_end.clear();
-    L.seg_equal.clear();
-    L.cells = 0;
-    L.nbuckets = 0;
-    L.max_ns = 0;
-}
-
-// Host arrays of one call, kept per thread so that their pages stay mapped
-// from call to call (first-touch page faults of fresh multi-MB vectors cost
-// more than the loops that fill them).
-struct PartHost {
-    Level l1, l2;
-    std::vector<int64_t> st1, st2, seg;
-    std::vector<MergeTile> tiles;
-    std::vector<int> list;
-    std::vector<SegItem> items;
-};
-
-void add_segment(Level& L, int64_t begin, int64_t end, int sp0, int ns, int lo, int hi, bool equal) {
-    const int64_t len = end - begin;
-    if (len <= 0) return;
-    const int64_t blocks = (len + kPartBlockElems - 1) / kPartBlockElems;
-    const int nb = 2 * ns + 1;
-    L.segs.push_back(PartSeg{L.cells, L.nbuckets, int32_t(blocks), nb, equal ? 1 : 0, 0});
-    L.seg_end.push_back(end);
-    L.seg_equal.push_back(equal ? 1 : 0);
-    for (int64_t k = 0; k < blocks; ++k) {
-        const int64_t b0 = begin + k * kPartBlockElems;
-        L.work.push_back(PartBlock{b0, std::min(end, b0 + kPartBlockElems), L.cells + k, int32_t(blocks), sp0, ns,
-                                   lo, hi, 0});
-    }
-    L.cells += int64_t(nb) * blocks;
-    L.nbuckets += nb;
-    L.max_ns = std::max(L.max_ns, ns);
-}
-
-// Count, scan and scatter of one level (src -> dst) and its bucket starts in
-// d_starts (nbuckets + 1 entries); with h_starts they are also read back
-// (one host synchronisation).
-int run_level(const PartKernels& pk, const Level& L, const void* src, const void* splitters, void* dst,
-              int64_t* d_starts, unsigned char* d_equal, std::vector<int64_t>* h_starts, Scratch& tmp) {
-    const hipStream_t stream = tmp.stream();
-    PartBlock* work = nullptr;
-    PartSeg* segs = nullptr;
-    int64_t *counts = nullptr, *sums = nullptr;
-    int rc = tmp.get(&work, L.work.size());
-    if (rc == CS_OK) rc = tmp.get(&segs, L.segs.size());
-    if (rc == CS_OK) rc = tmp.get(&counts, size_t(L.cells));
-    if (rc == CS_OK) rc = tmp.get(&sums, size_t((L.cells + kScanTile - 1) / kScanTile) + 1);
-    if (rc != CS_OK) return rc;
-    hipError_t e = hipMemcpyAsync(work, L.work.data(), L.work.size() * sizeof(PartBlock), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(segs, L.segs.data(), L.segs.size() * sizeof(PartSeg), hipMemcpyHostToDevice, stream);
-    const int blocks = int(L.work.size());
-    if (e == hipSuccess) e = pk.count(src, splitters, work, blocks, L.max_ns, counts, stream);
-    if (e == hipSuccess) e = exclusive_scan(counts, L.cells, sums, stream);
-    if (e == hipSuccess) e = pk.scatter(src, splitters, work, blocks, L.max_ns, counts, dst, stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_part_starts, dim3(unsigned(L.segs.size())), dim3(256), 0, stream, segs, counts, d_starts,
-                           d_equal);
-        e = hipGetLastError();
-    }
-    if (h_starts) {
-        h_starts->assign(size_t(L.nbuckets), 0);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(h_starts->data(), d_starts, h_starts->size() * sizeof(int64_t), hipMemcpyDeviceToHost,
-                               stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    }
-    return e == hipSuccess ? CS_OK : int(e);
-}
 
 // Parts of a partition sort of n elements (the caller's MinNbParts /
 // MaxNbParts bound them when nonzero): p1 first-level x p2 second-level
@@ -364,11 +373,10 @@ PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
 // out) with p1 x p2 parts taken from one global splitter list, so the second
 // level needs no sampling of its own.  The buckets are then sorted as one
 // ragged batch (values -> out, or in place in out), planned on the device;
-// the rare bucket longer than a tile sends the call to a host-planned batch
-// that cuts it into chunks merged by tiled passes.  Host synchronisations:
-// first-level bucket starts (two levels only) and the final plan's header;
-// the host-planned path reads all bucket starts and synchronises once more at
-// its end.
+// the rare bucket longer than a tile is sorted by k_part_long's chunk sort
+// and merge passes.  Everything is laid out on the device (the second
+// level's work list, the final plan, the long buckets' chunks and tiles):
+// the call enqueues its work on the stream and returns without waiting.
 int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, size_t values_elems, int min_parts,
                    int max_parts, Scratch& tmp) {
     const LargeKernels lk = large_kernels_for(dtype);
@@ -381,192 +389,142 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     const PartShape sh = part_shape(dtype, n, min_parts, max_parts);
     const int p1 = sh.p1, p2 = sh.p2;
     const int64_t parts = sh.parts, m = sh.m, target = sh.target;
+    const int64_t blocks1 = (n + kPartBlockElems - 1) / kPartBlockElems;
+    const int nb1 = 2 * p1 - 1;
+    const int64_t cells1 = int64_t(nb1) * blocks1;
+    // long-bucket sort (final buckets longer than the tile)
+    const int64_t nlong_max = n / lk.chunk + 1;
+    const int64_t ntile_max = n / lk.tile + nlong_max;
 
     unsigned char *samples = nullptr, *samples_tmp = nullptr, *gsp = nullptr, *l1sp = nullptr;
+    int64_t* cor = nullptr;
+    PartBlock* work1 = nullptr;
+    PartSeg* segs1 = nullptr;
+    int64_t *counts1 = nullptr, *sums1 = nullptr;
     int rc = tmp.get(&samples, size_t(m) * esz);
     if (rc == CS_OK) rc = tmp.get(&samples_tmp, size_t(m) * esz);
     if (rc == CS_OK) rc = tmp.get(&gsp, size_t(parts - 1) * size_t(pk.key_size));
     if (rc == CS_OK && p2 > 1) rc = tmp.get(&l1sp, size_t(std::max(p1 - 1, 1)) * size_t(pk.key_size));
+    if (rc == CS_OK) rc = tmp.get(&cor, size_t(ntile_max) + 1);
+    if (rc == CS_OK) rc = tmp.get(&work1, size_t(blocks1));
+    if (rc == CS_OK) rc = tmp.get(&segs1, 1);
+    if (rc == CS_OK) rc = tmp.get(&counts1, size_t(cells1));
+    if (rc == CS_OK) rc = tmp.get(&sums1, size_t((cells1 + kScanTile - 1) / kScanTile) + 1);
     if (rc != CS_OK) return rc;
 
-    // splitters from a sorted hashed sample; the first level's are every
-    // p2-th of them
+    // hashed sample sorted by the merge path; splitters and the first
+    // level's work list from it
     hipError_t e = pk.sample(data, n, samples, int(m), stream);
     if (e != hipSuccess) return int(e);
     rc = merge_sort(dtype, samples, m, samples, samples_tmp, tmp);
     if (rc != CS_OK) return rc;
-    e = pk.splitters(samples, int(m), int(parts), gsp, stream);
-    if (e == hipSuccess && p2 > 1) e = pk.gather(gsp, p1, p2, l1sp, stream);
+    if (e == hipSuccess)
+        e = pk.prep(samples, int(m), int(parts), p1, p2, gsp, l1sp, n, kPartBlockElems, blocks1, work1, segs1, stream);
     if (e != hipSuccess) return int(e);
 
-    // level 1: data -> values
-    thread_local PartHost host;
-    Level& L1 = host.l1;
-    Level& L2 = host.l2;
-    clear_level(L1);
-    clear_level(L2);
-    add_segment(L1, 0, n, 0, p1 - 1, -1, -1, false);
-    std::vector<int64_t>& st1 = host.st1;
-    int64_t *d_st1 = nullptr, *d_st2 = nullptr;
-    unsigned char *d_eq1 = nullptr, *d_eq2 = nullptr;
-    rc = tmp.get(&d_st1, size_t(L1.nbuckets) + 1);
-    if (rc == CS_OK) rc = tmp.get(&d_eq1, size_t(L1.nbuckets));
-    if (rc == CS_OK)
-        rc = run_level(pk, L1, data, p2 > 1 ? l1sp : gsp, values, d_st1, d_eq1, p2 > 1 ? &st1 : nullptr, tmp);
-    if (rc != CS_OK) return rc;
-
-    // level 2: values -> out, one segment per first-level bucket; equality
-    // buckets and buckets already small are copied through
-    const Level* fin = &L1;
-    const std::vector<int64_t>* fst = &st1;
-    std::vector<int64_t>& st2 = host.st2;
-    void* src = values;
-    if (p2 > 1) {
-        const int nb1 = 2 * p1 - 1;
-        for (int b = 0; b < nb1; ++b) {
-            const int64_t b0 = st1[size_t(b)], b1 = b + 1 < nb1 ? st1[size_t(b) + 1] : n;
-            const int j = b / 2;
-            if ((b & 1) || b1 - b0 <= target) {
-                add_segment(L2, b0, b1, 0, 0, -1, -1, (b & 1) != 0);
-            } else {
-                add_segment(L2, b0, b1, j * p2, p2 - 1, j > 0 ? j * p2 - 1 : -1,
-                            j < p1 - 1 ? (j + 1) * p2 - 1 : -1, false);
-            }
-        }
-        rc = tmp.get(&d_st2, size_t(L2.nbuckets) + 1);
-        if (rc == CS_OK) rc = tmp.get(&d_eq2, size_t(L2.nbuckets));
-        if (rc == CS_OK) rc = run_level(pk, L2, values, gsp, out, d_st2, d_eq2, nullptr, tmp);
+    // level 1: data -> values (straight into out for one level of an
+    // out-of-place sort), one segment over the whole array
+    const void* sp1 = p2 > 1 ? static_cast<const void*>(l1sp) : static_cast<const void*>(gsp);
+    void* dst1 = (p2 == 1 && out != data) ? out : values;
+    e = pk.count(data, sp1, work1, int(blocks1), p1 - 1, counts1, nullptr, stream);
+    if (e == hipSuccess) e = exclusive_scan(counts1, cells1, sums1, stream);
+    if (e == hipSuccess) e = pk.scatter(data, sp1, work1, int(blocks1), p1 - 1, counts1, dst1, nullptr, stream);
+    // one level in place: the buckets go back into out, so that the final
+    // sort also runs in place (equality buckets then stay as they are)
+    if (e != hipSuccess) return int(e);
+    if (p2 == 1 && dst1 != out) {
+        rc = cs_device_copy(out, values, size_t(n) * esz, stream);
         if (rc != CS_OK) return rc;
-        fin = &L2;
-        fst = &st2;
-        src = out;
     }
-    int64_t* d_fst = p2 > 1 ? d_st2 : d_st1;
-    const unsigned char* d_feq = p2 > 1 ? d_eq2 : d_eq1;
-    const int64_t fnb = fin->nbuckets;
 
-    // plan the final ragged sort on the device; only its header comes back
-    {
-        const SegClasses sc = seg_classes(dtype);
-        PlanTable t{};
-        t.maxb = sc.maxb;
-        t.small_max = sc.small_max;
-        for (int c = 0; c < kPlanClasses; ++c) {
-            t.spi[c] = c <= sc.maxb ? std::max(1, sc.spi[c]) : 1;
-            t.variant[c] = c <= sc.maxb ? sc.variant[c] : 0;
-        }
-        unsigned char* cls = nullptr;
-        int *meta = nullptr, *d_list = nullptr, *d_counters = nullptr;
-        SegItem* d_items = nullptr;
-        rc = tmp.get(&cls, size_t(fnb));
-        if (rc == CS_OK) rc = tmp.get(&meta, kPlanMeta);
-        if (rc == CS_OK) rc = tmp.get(&d_list, size_t(fnb));
-        if (rc == CS_OK) rc = tmp.get(&d_items, size_t(fnb));
-        if (rc == CS_OK) rc = tmp.get(&d_counters, 2);
+    // final plan's counters (meta, then the persistent kernels' two), cleared
+    // by the last level's bucket-starts kernel
+    constexpr int kZero = kPlanMeta + 2;
+    int* meta = nullptr;
+    rc = tmp.get(&meta, kZero);
+    if (rc != CS_OK) return rc;
+    int* d_counters = meta + kPlanMeta;
+    int64_t* d_fst = nullptr;
+    unsigned char* d_feq = nullptr;
+    const int64_t* d_fnb = nullptr;   // final bucket count on the device (two levels)
+    int64_t fnb = nb1;                // its bound
+    if (p2 == 1) {
+        rc = tmp.get(&d_fst, size_t(nb1) + 1);
+        if (rc == CS_OK) rc = tmp.get(&d_feq, size_t(nb1));
         if (rc != CS_OK) return rc;
-        const unsigned g = unsigned((fnb + 255) / 256);
-        hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(128), 0, stream, meta);
-        // equality buckets need no sort, but an out-of-place final sort must still move them
-        const int skip_equal = src == out ? 1 : 0;
-        hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, d_feq, skip_equal, fnb, n, lk.chunk,
-                           cls, meta);
-        hipLaunchKernelGGL(k_plan_layout, dim3(1), dim3(64), 0, stream, meta, t);
-        hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls, fnb, meta, d_list);
-        hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta, t, d_items);
-        int hdr[3] = {0, 0, 0};
+        hipLaunchKernelGGL(k_part_starts, dim3(1), dim3(256), 0, stream, segs1, counts1, d_fst, d_feq, meta, kZero);
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(hdr, meta + 4 * kPlanClasses, sizeof(hdr), hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return int(e);
-        if (hdr[2] == 0) {   // no bucket longer than the tile: sort with the device plan
-            SegShape ss;
-            ss.grid[kSegMulti] = seg_persistent_grid(dtype, kSegMulti, tmp.grid_cap);
-            ss.grid[kSegSingle] = seg_persistent_grid(dtype, kSegSingle, tmp.grid_cap);
-            if (ss.grid[kSegMulti] <= 0 || ss.grid[kSegSingle] <= 0) return CS_E_NO_DEVICE;
-            e = seg_launch_all(dtype, ss, src, out, d_fst, d_list, d_items, 0, d_counters, stream,
-                               meta + 4 * kPlanClasses);
-            return e == hipSuccess ? CS_OK : int(e);
+    } else {
+        // level 2: values -> out, one segment per first-level bucket, laid
+        // out on the device; grids are sized by the bounds and the surplus
+        // workgroups return at once
+        const int64_t nbs = 2 * int64_t(p2) - 1;
+        const int64_t blocks2 = blocks1 + nb1;   // every segment rounds up once
+        const int64_t cells2 = nbs * blocks2;
+        fnb = nbs * nb1;
+        PartBlock* work2 = nullptr;
+        PartSeg* segs2 = nullptr;
+        int64_t *hdr2 = nullptr, *counts2 = nullptr, *sums2 = nullptr;
+        rc = tmp.get(&work2, size_t(blocks2));
+        if (rc == CS_OK) rc = tmp.get(&segs2, size_t(nb1));
+        if (rc == CS_OK) rc = tmp.get(&hdr2, 4);
+        if (rc == CS_OK) rc = tmp.get(&counts2, size_t(cells2));
+        if (rc == CS_OK) rc = tmp.get(&sums2, size_t((cells2 + kScanTile - 1) / kScanTile) + 1);
+        if (rc == CS_OK) rc = tmp.get(&d_fst, size_t(fnb) + 1);
+        if (rc == CS_OK) rc = tmp.get(&d_feq, size_t(fnb));
+        if (rc != CS_OK) return rc;
+        hipLaunchKernelGGL(k_part_level2, dim3(1), dim3(kL2Threads), 0, stream, counts1, blocks1, nb1, n, p1, p2,
+                           target, kPartBlockElems, work2, segs2, hdr2);
+        e = hipGetLastError();
+        // cells past hdr2[1] are scanned too; no used cell depends on them
+        if (e == hipSuccess) e = pk.count(values, gsp, work2, int(blocks2), p2 - 1, counts2, hdr2, stream);
+        if (e == hipSuccess) e = exclusive_scan(counts2, cells2, sums2, stream);
+        if (e == hipSuccess) e = pk.scatter(values, gsp, work2, int(blocks2), p2 - 1, counts2, out, hdr2, stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_part_starts, dim3(unsigned(nb1)), dim3(256), 0, stream, segs2, counts2, d_fst, d_feq,
+                               meta, kZero);
+            e = hipGetLastError();
         }
+        d_fnb = hdr2 + 2;
     }
-    // some bucket is longer than the tile: the host plans the sort and
-    // cuts the long buckets into chunks
-    std::vector<int64_t>& fst_host = p2 > 1 ? st2 : st1;
-    fst_host.assign(size_t(fnb), 0);
-    e = hipMemcpyAsync(fst_host.data(), d_fst, size_t(fnb) * sizeof(int64_t), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return int(e);
 
-    // one ragged batch over all buckets (src -> out); long buckets cut into
-    // chunks.  The level's bucket starts run through the array in order.
-    const int64_t chunk = lk.chunk;
-    const std::vector<int64_t>& st = *fst;
-    std::vector<int64_t>& seg = host.seg;
-    seg.assign(1, 0);
-    seg.reserve(st.size() / 2 + size_t(n / chunk) + 2);
-    std::vector<MergeTile>& tiles = host.tiles;
-    tiles.clear();
-    int passes = 0;
-    size_t g = 0;   // segment of bucket i (only needed for long buckets)
-    for (size_t i = 0; i < st.size(); ++i) {
-        const int64_t b0 = st[i], b1 = i + 1 < st.size() ? st[i + 1] : n;
-        const int64_t len = b1 - b0;
-        if (len <= 0) continue;
-        if (b0 != seg.back()) return CS_E_NO_DEVICE;   // buckets must tile the array in order
-        if (len <= chunk) {
-            seg.push_back(b1);
-            continue;
-        }
-        for (int64_t c = 0; c < len; c += chunk) seg.push_back(b0 + std::min(len, c + chunk));
-        while (g + 1 < fin->segs.size() && fin->segs[g + 1].first <= int64_t(i)) ++g;
-        const bool equal = fin->seg_equal[g] || ((int64_t(i) - fin->segs[g].first) & 1);
-        if (!equal) {   // equality buckets are sorted once their chunks are
-            int p = 0;
-            for (int64_t run = chunk; run < len; run *= 2) ++p;
-            passes = std::max(passes, p);
-            for (int64_t d0 = 0; d0 < len; d0 += lk.tile) tiles.push_back(MergeTile{b0, len, d0});
-        }
+    // plan the final ragged sort on the device
+    const SegClasses sc = seg_classes(dtype);
+    PlanTable t{};
+    t.maxb = sc.maxb;
+    t.small_max = sc.small_max;
+    for (int c = 0; c < kPlanClasses; ++c) {
+        t.spi[c] = c <= sc.maxb ? std::max(1, sc.spi[c]) : 1;
+        t.variant[c] = c <= sc.maxb ? sc.variant[c] : 0;
     }
-    if (seg.back() != n) return CS_E_NO_DEVICE;
-    std::vector<int>& list = host.list;
-    std::vector<SegItem>& items = host.items;
-    const int64_t nseg = int64_t(seg.size()) - 1;
-    rc = seg_build_items(dtype, seg.data(), nseg, nullptr, list, items, nullptr, nullptr, nullptr);
-    if (rc != CS_OK) return rc;
-    const SegShape shape = seg_shape(dtype, items);
-    int64_t* d_seg = nullptr;
-    int *d_list = nullptr, *d_counters = nullptr;
+    unsigned char* cls = nullptr;
+    int *d_list = nullptr, *long_list = nullptr;
     SegItem* d_items = nullptr;
-    MergeTile* d_tiles = nullptr;
-    rc = tmp.get(&d_seg, seg.size());
-    if (rc == CS_OK) rc = tmp.get(&d_list, list.size());
-    if (rc == CS_OK) rc = tmp.get(&d_items, items.size());
-    if (rc == CS_OK) rc = tmp.get(&d_counters, 2);
-    if (rc == CS_OK && !tiles.empty()) rc = tmp.get(&d_tiles, tiles.size());
+    rc = tmp.get(&cls, size_t(fnb));
+    if (rc == CS_OK) rc = tmp.get(&d_list, size_t(fnb));
+    if (rc == CS_OK) rc = tmp.get(&d_items, size_t(fnb));
+    if (rc == CS_OK) rc = tmp.get(&long_list, size_t(nlong_max));
     if (rc != CS_OK) return rc;
-    e = hipMemcpyAsync(d_seg, seg.data(), seg.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, stream);
+    const unsigned g = unsigned((fnb + 255) / 256);
+    // the final sort runs in place: equality buckets need no sort
+    hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, d_feq, 1, fnb, d_fnb, n,
+                       int64_t(lk.chunk), cls, meta, long_list);
+    hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls, fnb, d_fnb, meta, t, d_list);
+    hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta, t, d_items);
+    e = hipGetLastError();
+    if (e != hipSuccess) return int(e);
+    SegShape ss;
+    ss.grid[kSegMulti] = seg_persistent_grid(dtype, kSegMulti, tmp.grid_cap);
+    ss.grid[kSegSingle] = seg_persistent_grid(dtype, kSegSingle, tmp.grid_cap);
+    if (ss.grid[kSegMulti] <= 0 || ss.grid[kSegSingle] <= 0) return CS_E_NO_DEVICE;
+    e = seg_launch_all(dtype, ss, out, out, d_fst, d_list, d_items, 0, d_counters, stream, meta + 4 * kPlanClasses,
+                       false);
+    // buckets longer than the tile (meta[kPlanLong] of them): chunk sorts and
+    // merge passes between out and values
     if (e == hipSuccess)
-        e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(SegItem), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess && !tiles.empty())
-        e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(MergeTile), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-        e = seg_launch_all(dtype, shape, src, out, d_seg, d_list, d_items, int(items.size()), d_counters, stream);
-    rc = e == hipSuccess ? CS_OK : int(e);
-    // merge passes over the long buckets (out <-> values), then final placement
-    void* a = out;
-    void* b2 = values;
-    for (int p = 0; rc == CS_OK && p < passes; ++p) {
-        e = pk.seg_merge(a, b2, d_tiles, int(tiles.size()), chunk << p, 0, stream);
-        if (e != hipSuccess) rc = int(e);
-        std::swap(a, b2);
-    }
-    if (rc == CS_OK && a != out) {
-        e = pk.seg_merge(a, out, d_tiles, int(tiles.size()), 0, 1, stream);
-        if (e != hipSuccess) rc = int(e);
-    }
-    const hipError_t se = hipStreamSynchronize(stream);   // host arrays above are read by the copies
-    if (rc == CS_OK && se != hipSuccess) rc = int(se);
-    return rc;
+        e = pk.long_sort(out, out, values, d_fst, long_list, meta + kPlanLong, cor, tmp.grid_cap, stream);
+    return e == hipSuccess ? CS_OK : int(e);
 }
 
 // Every temporary partition_sort can take for n elements, at its largest:
@@ -580,32 +538,32 @@ size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_part
     const PartShape sh = part_shape(dtype, n, min_parts, max_parts);
     auto r = [](size_t bytes) { return scratch_round(bytes ? bytes : 1); };
     size_t total = 0;
+    const int64_t nlong_max = n / lk.chunk + 1;
+    const int64_t ntile_max = n / lk.tile + nlong_max;
     total += 2 * r(size_t(sh.m) * esz);                                   // samples, sorted copy
     total += merge_scratch_bytes(dtype, sh.m);                            // their merge sort
     total += r(size_t(sh.parts - 1) * size_t(pk.key_size));               // global splitters
     total += r(size_t(std::max(sh.p1 - 1, 1)) * size_t(pk.key_size));     // first-level splitters
-    auto level = [&](int64_t segs, int64_t blocks, int64_t nbuckets, int64_t cells) {
-        size_t b = r(size_t(nbuckets + 1) * 8) + r(size_t(nbuckets));     // starts, equality flags
-        b += r(size_t(blocks) * sizeof(PartBlock)) + r(size_t(segs) * sizeof(PartSeg));
+    total += r(size_t(ntile_max + 1) * 8) + r(size_t(nlong_max) * 4);     // long buckets: co-ranks, list
+    total += r(size_t(kPlanMeta + 2) * 4);                                // plan counters
+    auto level = [&](int64_t segs, int64_t blocks, int64_t cells) {
+        size_t b = r(size_t(blocks) * sizeof(PartBlock)) + r(size_t(segs) * sizeof(PartSeg));
         b += r(size_t(cells) * 8) + r(size_t((cells + kScanTile - 1) / kScanTile + 1) * 8);
         return b;
     };
     const int64_t blocks1 = (n + kPartBlockElems - 1) / kPartBlockElems;
     const int64_t nb1 = 2 * int64_t(sh.p1) - 1;
-    total += level(1, blocks1, nb1, nb1 * blocks1);
+    total += level(1, blocks1, nb1 * blocks1);
     int64_t fnb = nb1;
     if (sh.p2 > 1) {
         const int64_t nbs = 2 * int64_t(sh.p2) - 1;                       // buckets per second-level segment
         const int64_t blocks2 = blocks1 + nb1;                            // each segment rounds up once
         fnb = nbs * nb1;
-        total += level(nb1, blocks2, fnb, nbs * blocks2);
+        total += level(nb1, blocks2, nbs * blocks2) + r(4 * 8);           // + device header
     }
+    total += r(size_t(fnb + 1) * 8) + r(size_t(fnb));                     // final starts, equality flags
     // device plan of the final sort
-    total += r(size_t(fnb)) + r(size_t(kPlanMeta) * 4) + r(size_t(fnb) * 4) + r(size_t(fnb) * sizeof(SegItem)) + r(8);
-    // host-planned fallback: every bucket plus every chunk of a long one
-    const int64_t segs = fnb + n / lk.chunk + 2;
-    total += r(size_t(segs) * 8) + r(size_t(segs) * 4) + r(size_t(segs) * sizeof(SegItem)) + r(8);
-    total += r(size_t(n / lk.tile + fnb + 1) * sizeof(MergeTile));
+    total += r(size_t(fnb)) + r(size_t(fnb) * 4) + r(size_t(fnb) * sizeof(SegItem));
     return total;
 }
 

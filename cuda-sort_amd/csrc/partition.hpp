@@ -23,6 +23,8 @@
 // sorted runs are merged by tiled merge passes over a tile list.
 #pragma once
 
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "large.hpp"
@@ -48,12 +50,6 @@ __global__ void k_part_sample(const typename Elem<DT>::Storage* src, int64_t n, 
     }
 }
 
-template <int DT>
-__global__ void k_part_splitters(const typename Elem<DT>::Storage* sorted_samples, int m, int parts,
-                                 typename Elem<DT>::Key* splitters) {
-    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j < parts - 1) splitters[j] = Elem<DT>::key(sorted_samples[int64_t(j + 1) * m / parts - 1]);
-}
 
 // Elements per thread per tile of the count and scatter kernels.
 template <int DT> constexpr int part_items() { return sizeof(typename Elem<DT>::Storage) == 4 ? 16 : 8; }
@@ -96,6 +92,27 @@ struct PartSeg {
     int32_t equal;   // every bucket of the segment holds one key value (a first-level equality bucket)
     int32_t pad;
 };
+
+// Splitters and the first level's work list, from the sorted sample: the
+// P - 1 global splitters are every (m / P)-th sample; with two levels the
+// p1 - 1 first-level splitters are every p2-th global splitter (global
+// splitter (j + 1) p2 - 1), so the second level of first-level bucket j uses
+// the p2 - 1 global splitters inside it.  The first level is one segment
+// over the whole array cut into workgroups of block_elems elements.
+template <int DT>
+__global__ void k_part_prep(const typename Elem<DT>::Storage* sorted, int m, int parts, int p1, int p2,
+                            typename Elem<DT>::Key* gsp, typename Elem<DT>::Key* l1sp, int64_t n,
+                            int64_t block_elems, int64_t blocks1, PartBlock* work1, PartSeg* segs1) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < parts - 1) gsp[j] = Elem<DT>::key(sorted[(j + 1) * m / parts - 1]);
+    if (p2 > 1 && j < p1 - 1) l1sp[j] = Elem<DT>::key(sorted[(j + 1) * p2 * m / parts - 1]);
+    const int ns = p1 - 1;
+    if (j == 0) segs1[0] = PartSeg{0, 0, int32_t(blocks1), 2 * ns + 1, 0, 0};
+    if (j < blocks1) {
+        const int64_t x0 = j * block_elems;
+        work1[j] = PartBlock{x0, x0 + block_elems < n ? x0 + block_elems : n, j, int32_t(blocks1), 0, ns, -1, -1, 0};
+    }
+}
 
 // Classification.  A lookup table indexed by kPartLutBits bits of the key's
 // order-preserving unsigned image, taken relative to the segment's lower
@@ -219,8 +236,9 @@ template <int DT> size_t part_count_lds(int ns) {
 template <int DT>
 __global__ void __launch_bounds__(kPartThreads)
 k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
-             int max_ns, int64_t* counts) {
+             int max_ns, int64_t* counts, const int64_t* nwork) {
     using E = Elem<DT>;
+    if (nwork && int64_t(blockIdx.x) >= *nwork) return;   // grid sized by a bound, work list built on the device
     constexpr int IT = part_items<DT>();
     extern __shared__ __align__(16) unsigned char part_lds[];
     typename E::Key* sp = reinterpret_cast<typename E::Key*>(part_lds);
@@ -312,8 +330,9 @@ template <int DT> size_t part_scatter_lds(int ns) {
 template <int DT>
 __global__ void __launch_bounds__(part_scatter_threads<DT>())
 k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
-               int max_ns, const int64_t* offsets, typename Elem<DT>::Storage* dst) {
+               int max_ns, const int64_t* offsets, typename Elem<DT>::Storage* dst, const int64_t* nwork) {
     using E = Elem<DT>;
+    if (nwork && int64_t(blockIdx.x) >= *nwork) return;   // grid sized by a bound, work list built on the device
     using K = typename E::Key;
     using S = typename E::Storage;
     constexpr int IT = part_items<DT>();
@@ -434,15 +453,10 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
 // Bucket starts of every segment (one workgroup per segment), and whether
 // each bucket is an equality bucket (every key in it equal: already sorted,
 // tie order being free).
-__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts, unsigned char* equal);
+// Workgroup 0 also clears zero[0, nzero) (the final plan's counters).
+__global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64_t* starts, unsigned char* equal,
+                              int* zero, int nzero);
 
-// The second level's splitters of first-level bucket j are every parts2-th
-// global splitter: first-level splitter j = global splitter (j+1)*parts2 - 1.
-template <class K>
-__global__ void k_part_gather(const K* global, int parts1, int parts2, K* level1) {
-    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j < parts1 - 1) level1[j] = global[int64_t(j + 1) * parts2 - 1];
-}
 
 // Exclusive scan of int64 values in place, three launches: per-block sums,
 // a single-block scan of those sums, and the per-block scan with its offset.
@@ -497,14 +511,116 @@ k_seg_merge_pass(const typename Elem<DT>::Storage* src, typename Elem<DT>::Stora
                         merge_end_corank(g, &items[blockIdx.x + 1].a0));
 }
 
+
+// Buckets longer than the on-chip tile (keys crowded into a range the sample
+// missed; equality buckets never get here, the final sort runs in place and
+// leaves them as they are) are sorted without the host knowing whether any
+// exist, by two ordinary launches that return at once when there are none:
+//  * k_part_long_chunks: the chunk sorts of every long bucket, chunk q of the
+//    running count over the buckets going to workgroup q mod grid (src ->
+//    out, the chunk kernel's network);
+//  * k_part_long_merge: one workgroup per long bucket runs all of its merge
+//    passes (out <-> other) and the copy back after an odd number of them:
+//    per pass it finds the co-ranks of all the bucket's merge tiles (one
+//    search per thread, all in flight together), then merges the tiles in
+//    turn.  Only workgroup barriers: no workgroup ever waits for another.
+// A sample that misses a crowded key range leaves a few buckets just over
+// the tile; only an input built against the hashed sample positions would
+// make one long bucket large, and that bucket is then merged by one CU.
+// long_list[0, *nlong): the long buckets, any order; st: bucket starts.
+template <int DT>
+__global__ void __launch_bounds__(kMergeThreads)
+k_part_long_chunks(const typename Elem<DT>::Storage* src, typename Elem<DT>::Storage* out, const int64_t* st,
+                   const int* long_list, const int* nlong_p) {
+    using E = Elem<DT>;
+    constexpr int LW = chunk_logw<DT>();
+    constexpr int LE = auto_loge(DT, LW);
+    using C = TileCfg<E, LW, LE>;
+    static_assert(C::BLOCK == kMergeThreads && C::APB == 1, "one chunk per workgroup of the merge width");
+    constexpr int64_t CHUNK = int64_t(1) << LW;
+    __shared__ typename E::Key lds_k[C::LDS_KEYS];
+    __shared__ int32_t lds_p[C::LDS_PAYLOAD];
+    const int nlong = *nlong_p;
+    const int64_t G = gridDim.x, me = blockIdx.x;
+    int64_t base = 0;   // chunks of the buckets before l (the same in every workgroup)
+    for (int l = 0; l < nlong; ++l) {
+        const int b = long_list[l];
+        const int64_t b0 = st[b], len = st[b + 1] - b0;
+        const int64_t nch = (len + CHUNK - 1) / CHUNK;
+        for (int64_t c = ((me - base) % G + G) % G; c < nch; c += G) {
+            const int64_t c0 = c * CHUNK;
+            const int clen = int(len - c0 < CHUNK ? len - c0 : CHUNK);
+            tile_sort<E, LW, LE>(src + b0 + c0, out + b0 + c0, clen, int(threadIdx.x), lds_k, lds_p);
+            __syncthreads();   // the tile's LDS is reused by the next chunk
+        }
+        base += nch;
+    }
+}
+
+// cor: scratch of one int64 per merge tile of all long buckets together
+// (tile_base: the tiles of the buckets before l, the same running count in
+// every workgroup).
+template <int DT>
+__global__ void __launch_bounds__(kMergeThreads)
+k_part_long_merge(typename Elem<DT>::Storage* out, typename Elem<DT>::Storage* other, const int64_t* st,
+                  const int* long_list, const int* nlong_p, int64_t* cor) {
+    using E = Elem<DT>;
+    using S = typename E::Storage;
+    constexpr int64_t CHUNK = int64_t(1) << chunk_logw<DT>();
+    constexpr int TILE = merge_tile<DT>();
+    const int nlong = *nlong_p;
+    const int tid = int(threadIdx.x);
+    int64_t tile_base = 0;
+    for (int l = 0; l < nlong; ++l) {
+        const int b = long_list[l];
+        const int64_t b0 = st[b], len = st[b + 1] - b0;
+        const int64_t ntiles = (len + TILE - 1) / TILE;
+        if (l % int(gridDim.x) == int(blockIdx.x)) {
+            int64_t* tc = cor + tile_base;
+            S* a = out + b0;
+            S* b2 = other + b0;
+            for (int64_t run = CHUNK; run < len; run *= 2) {
+                for (int64_t q = tid; q < ntiles; q += kMergeThreads) {
+                    const MergeGeom g = merge_geom<TILE>(len, run, q * TILE);
+                    const S* A = a + g.base;
+                    tc[q] = thread_corank<E>(A, g.na, A + g.na, g.nb, g.d0);
+                }
+                __syncthreads();
+                for (int64_t q = 0; q < ntiles; ++q) {
+                    const MergeGeom g = merge_geom<TILE>(len, run, q * TILE);
+                    // a tile that does not end its run pair finds its end
+                    // co-rank as the next tile's start
+                    merge_tile_body<DT>(a, b2, len, run, q * TILE, tc[q], merge_end_corank(g, tc + q + 1));
+                    __syncthreads();   // LDS reused by the next tile
+                }
+                S* t = a;
+                a = b2;
+                b2 = t;
+            }
+            if (a != out + b0)   // odd number of passes: back into out
+                for (int64_t i = tid; i < len; i += kMergeThreads) out[b0 + i] = a[i];
+            __syncthreads();
+        }
+        tile_base += ntiles;
+    }
+}
+
 struct PartKernels {
     hipError_t (*sample)(const void* src, int64_t n, void* samples, int m, hipStream_t s) = nullptr;
-    hipError_t (*splitters)(const void* sorted_samples, int m, int parts, void* splitters, hipStream_t s) = nullptr;
+    // k_part_prep
+    hipError_t (*prep)(const void* sorted_samples, int m, int parts, int p1, int p2, void* gsp, void* l1sp, int64_t n,
+                       int64_t block_elems, int64_t blocks1, PartBlock* work1, PartSeg* segs1,
+                       hipStream_t s) = nullptr;
+    // count / scatter over `blocks` work items; with nwork (device) only the
+    // first *nwork of them exist and the rest of the grid returns at once.
     hipError_t (*count)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
-                        int64_t* counts, hipStream_t s) = nullptr;
+                        int64_t* counts, const int64_t* nwork, hipStream_t s) = nullptr;
     hipError_t (*scatter)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
-                          const int64_t* offsets, void* dst, hipStream_t s) = nullptr;
-    hipError_t (*gather)(const void* global, int parts1, int parts2, void* level1, hipStream_t s) = nullptr;
+                          const int64_t* offsets, void* dst, const int64_t* nwork, hipStream_t s) = nullptr;
+    // k_part_long_chunks + k_part_long_merge over the long buckets of a
+    // device plan (grids of at most grid_cap workgroups).
+    hipError_t (*long_sort)(const void* src, void* out, void* other, const int64_t* st, const int* long_list,
+                            const int* nlong, int64_t* cor, int grid_cap, hipStream_t s) = nullptr;
     // Co-rank search + merge pass (or a plain copy with copy_only); writes
     // the co-ranks into items.
     hipError_t (*seg_merge)(const void* src, void* dst, MergeTile* items, int nitems, int64_t run,
@@ -524,19 +640,22 @@ PartKernels part_kernels() {
                            static_cast<const S*>(src), n, static_cast<S*>(samples), m);
         return hipGetLastError();
     };
-    k.splitters = [](const void* sorted, int m, int parts, void* sp, hipStream_t s) {
-        hipLaunchKernelGGL(k_part_splitters<DT>, dim3(unsigned((parts + 255) / 256)), dim3(256), 0, s,
-                           static_cast<const S*>(sorted), m, parts, static_cast<K*>(sp));
+    k.prep = [](const void* sorted, int m, int parts, int p1, int p2, void* gsp, void* l1sp, int64_t n,
+                int64_t block_elems, int64_t blocks1, PartBlock* work1, PartSeg* segs1, hipStream_t s) {
+        const int64_t threads = std::max<int64_t>(parts, blocks1);
+        hipLaunchKernelGGL(k_part_prep<DT>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const S*>(sorted), m, parts, p1, p2, static_cast<K*>(gsp),
+                           static_cast<K*>(l1sp), n, block_elems, blocks1, work1, segs1);
         return hipGetLastError();
     };
     k.count = [](const void* src, const void* sp, const PartBlock* work, int blocks, int max_ns, int64_t* counts,
-                 hipStream_t s) {
+                 const int64_t* nwork, hipStream_t s) {
         hipLaunchKernelGGL(k_part_count<DT>, dim3(unsigned(blocks)), dim3(kPartThreads), part_count_lds<DT>(max_ns),
-                           s, static_cast<const S*>(src), static_cast<const K*>(sp), work, max_ns, counts);
+                           s, static_cast<const S*>(src), static_cast<const K*>(sp), work, max_ns, counts, nwork);
         return hipGetLastError();
     };
     k.scatter = [](const void* src, const void* sp, const PartBlock* work, int blocks, int max_ns,
-                   const int64_t* offsets, void* dst, hipStream_t s) {
+                   const int64_t* offsets, void* dst, const int64_t* nwork, hipStream_t s) {
         const size_t lds = part_scatter_lds<DT>(max_ns);
         if (lds > 64 * 1024) {
             const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_scatter<DT>),
@@ -545,13 +664,24 @@ PartKernels part_kernels() {
         }
         hipLaunchKernelGGL(k_part_scatter<DT>, dim3(unsigned(blocks)), dim3(part_scatter_threads<DT>()), lds, s,
                            static_cast<const S*>(src), static_cast<const K*>(sp), work, max_ns, offsets,
-                           static_cast<S*>(dst));
+                           static_cast<S*>(dst), nwork);
         return hipGetLastError();
     };
-    k.gather = [](const void* global, int parts1, int parts2, void* level1, hipStream_t s) {
-        if (parts1 < 2) return hipSuccess;
-        hipLaunchKernelGGL(k_part_gather<K>, dim3(unsigned((parts1 + 255) / 256)), dim3(256), 0, s,
-                           static_cast<const K*>(global), parts1, parts2, static_cast<K*>(level1));
+    k.long_sort = [](const void* src, void* out, void* other, const int64_t* st, const int* long_list,
+                     const int* nlong, int64_t* cor, int grid_cap, hipStream_t s) {
+        // one workgroup per CU for each kernel (any grid is correct)
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        int grid = std::max(1, cus);
+        if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
+        hipLaunchKernelGGL(k_part_long_chunks<DT>, dim3(unsigned(grid)), dim3(kMergeThreads), 0, s,
+                           static_cast<const S*>(src), static_cast<S*>(out), st, long_list, nlong);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_part_long_merge<DT>, dim3(unsigned(grid)), dim3(kMergeThreads), 0, s,
+                           static_cast<S*>(out), static_cast<S*>(other), st, long_list, nlong, cor);
         return hipGetLastError();
     };
     k.seg_merge = [](const void* src, void* dst, MergeTile* items, int nitems, int64_t run, int copy_only,

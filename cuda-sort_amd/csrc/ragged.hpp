@@ -57,9 +57,10 @@ SegShape seg_shape(int dtype, const std::vector<SegItem>& items);
 // Counter reset and both persistent launches, stream ordered.  With d_range
 // (two ints in device memory: items of the multi-wave kernel, all items)
 // the item ranges come from a plan built on the device and `sh` only gives
-// the grid sizes.
+// the grid sizes.  reset = false: the caller has already zeroed d_counters
+// in stream order.
 hipError_t seg_launch_all(int dtype, const SegShape& sh, const void* in, void* out, const int64_t* off,
                           const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s,
-                          const int* d_range = nullptr);
+                          const int* d_range = nullptr, bool reset = true);
 
 }  // namespace cs

@@ -339,9 +339,9 @@ SegShape seg_shape(int dtype, const std::vector<SegItem>& items) {
 
 hipError_t seg_launch_all(int dtype, const SegShape& sh, const void* in, void* out, const int64_t* off,
                           const int* d_list, const SegItem* d_items, int nitems, int* d_counters, hipStream_t s,
-                          const int* d_range) {
+                          const int* d_range, bool reset) {
     if (!d_range && nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counters);
+    if (reset) hipLaunchKernelGGL(k_seg_reset, dim3(1), dim3(64), 0, s, d_counters);
     hipError_t e = hipGetLastError();
     const SegKernels k = seg_kernels_for(dtype);
     if (e == hipSuccess)

@@ -201,12 +201,11 @@ int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant);
  * MinNbParts, MaxNbParts> (cusort.hpp:2020-2023, 1869-2017): sample-sort
  * partition into up to 2*max_parts-1 buckets (equality buckets for repeated
  * splitters), one ragged sort of all buckets, merge passes for buckets longer
- * than a chunk.  It synchronises the stream once or twice (first-level bucket
- * starts of a two-level partition, the final plan's 12-byte header) and once
- * more when a bucket is longer than a chunk; the reference synchronises per
- * recursion level (cusort.hpp:1977).  Without a workspace its temporaries are
- * stream-ordered allocations from a per-device pool that keeps freed memory
- * cached until the process ends.
+ * than a chunk.  Every step is laid out on the device, so the call only
+ * enqueues work on the stream and never synchronises it (the reference
+ * synchronises per recursion level, cusort.hpp:1977).  Without a workspace
+ * its temporaries are stream-ordered allocations from a per-device pool that
+ * keeps freed memory cached until the process ends.
  *
  * cs_sort_hybrid replaces HybridSort / HybridSortV3 (cusort.hpp:2656-2754):
  * merge below cs_hybrid_threshold(dtype) elements, partition above.

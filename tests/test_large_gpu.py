@@ -92,13 +92,27 @@ def test_config4_full(cs, orc, gpu, dtype, n, dist):
 def test_partition_level_boundaries(cs, orc, gpu, dtype):
     """Sizes around the one-level / two-level switch of the partition path
     (256 parts of 3/8 of the on-chip tile) and small part bounds that force
-    buckets longer than the tile onto the host-planned chunk + merge path."""
+    buckets longer than the tile onto the long-bucket chunk sort + merge passes."""
     dt = getattr(cs, dtype)
     target = (8192 if cs.DTYPE_SIZE[dt] == 4 else 4096) * 3 // 8
     for n in (256 * target - 1, 256 * target, 256 * target + 1, 300 * target + 17):
         run_large(cs, orc, gpu, dt, orc.D_UNIFORM, n, "partition", seed=n)
     run_large(cs, orc, gpu, dt, orc.D_DUP50, 300 * target + 17, "partition", max_parts=4)
     run_large(cs, orc, gpu, dt, orc.D_UNIFORM, 70000, "partition", min_parts=4000)
+
+
+@pytest.mark.parametrize("dtype", ["I32", "U64", "REC_I32_I32", "REC_I64_I32"])
+def test_partition_long_buckets_two_levels(cs, orc, gpu, dtype):
+    """A part bound above the one-level limit but far below n / target: two
+    levels laid out on the device whose final buckets are several tiles
+    long, so the cooperative chunk sort + merge passes run after the
+    persistent bucket sort (in place and through a workspace)."""
+    dt = getattr(cs, dtype)
+    n = (1 << 23) + 11 if cs.DTYPE_SIZE[dt] <= 8 else (1 << 22) + 11
+    for d in ("D_UNIFORM", "D_DUP50", "D_PRESORTED_RUNS"):
+        run_large(cs, orc, gpu, dt, getattr(orc, d), n, "partition", seed=n + 1, max_parts=300)
+    run_large(cs, orc, gpu, dt, orc.D_UNIFORM, n, "partition_ws", seed=n + 2, max_parts=300)
+    run_large(cs, orc, gpu, dt, orc.D_CONSTANT, n, "partition", seed=n + 3, max_parts=300)
 
 
 @pytest.mark.parametrize("dtype", ["F32", "F64"])
