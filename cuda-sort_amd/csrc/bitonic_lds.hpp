@@ -11,15 +11,11 @@
 // padding are the reference's (reflection first layer i ^ (2^k - 1), then
 // i ^ 2^j).
 //
-// Layers whose pairs stay inside one wave's positions run without a
-// workgroup barrier: thread q of a layer touches positions inside the
-// aligned block of 2 x (its 32- or 64-thread chunk) whenever the layer's
-// pair distance fits that block (stride d <= 32 / 64, reflection of a stage
-// k <= 6 / 7), so a run of such layers only needs the wave's own LDS
-// accesses kept in order (the reference keeps sub-warp strides inside a
-// warp, cusort.hpp:628-677, 712-737).  A barrier separates every layer that
-// crosses waves from its neighbours.  64-position blocks when the block
-// size is an odd multiple of 32 (a partial last wave), 128 otherwise.
+// One workgroup barrier follows every layer.  Running the layers whose
+// pairs stay inside one wave's positions without it (the reference keeps
+// sub-warp strides inside a warp, cusort.hpp:628-677, 712-737) was measured
+// slower on gfx950: 4096 x 1024 int32 at 160 / 192 / 224 threads took 89 /
+// 77 / 84 us against 82 / 69 / 75 us with a barrier after every layer.
 //
 // TESTSORTED: before the network and after every stage k >= 5 the block
 // checks whether the array is already sorted and stops if so.  A sorted
@@ -41,14 +37,6 @@ __device__ __forceinline__ void lds_cx(typename E::Key* lk, int32_t* lp, int i, 
         lk[i] = b; lk[j] = a;
         if constexpr (E::kPayload) { lp[i] = pb; lp[j] = pa; }
     }
-}
-
-__device__ __forceinline__ void lds_wave_order() {
-    // DS instructions of one wave execute in order: only the compiler must
-    // not move LDS accesses across this point.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <class E>
@@ -97,37 +85,23 @@ k_bitonic_lds(const typename E::Storage* in, typename E::Storage* out, int64_t a
 
     const bool test = (flags & CS_FLAG_TESTSORTED) != 0;
     bool done = test && lds_sorted<E>(lk, n, &sorted_flag);
-    // block of positions owned by one chunk of consecutive threads (see above)
-    const int own = (T % 64 == 0) ? 128 : 64;
-    int prev = 0;   // pair block of the previous layer (0: ordered by a barrier already)
-    auto order = [&](int blk) {
-        if (prev > 0) {
-            if (prev <= own && blk <= own) lds_wave_order();
-            else __syncthreads();
-        }
-        prev = blk;
-    };
     for (int k = 1; k <= logw && !done; ++k) {
         const int half = 1 << (k - 1);
-        order(half << 1);
         for (int q = tid; q < W / 2; q += T) {
             const int i = ((q >> (k - 1)) << k) | (q & (half - 1));
             lds_cx<E>(lk, lp, i, i ^ ((half << 1) - 1));
         }
+        __syncthreads();
         for (int d = half >> 1; d >= 1; d >>= 1) {
             const int ld = __builtin_ctz(d);
-            order(d << 1);
             for (int q = tid; q < W / 2; q += T) {
                 const int i = ((q >> ld) << (ld + 1)) | (q & (d - 1));
                 lds_cx<E>(lk, lp, i, i + d);
             }
+            __syncthreads();
         }
-        if (test && k >= 5 && k < logw) {
-            done = lds_sorted<E>(lk, n, &sorted_flag);   // begins with a barrier
-            prev = 0;
-        }
+        if (test && k >= 5 && k < logw) done = lds_sorted<E>(lk, n, &sorted_flag);
     }
-    __syncthreads();
 
     for (int i = tid; i < n; i += T) dst[i] = E::make(lk[i], E::kPayload ? lp[i] : 0);
 }

@@ -13,6 +13,12 @@
 #include "ragged.hpp"
 
 namespace cs {
+int pair_record_dtype(int key_dtype);
+hipError_t pairs_pack(int key_dtype, const void* keys, const int32_t* vals, void* rec, int64_t n, hipStream_t s);
+hipError_t pairs_unpack(int key_dtype, const void* rec, void* keys, int32_t* vals, int64_t n, hipStream_t s);
+}  // namespace cs
+
+namespace cs {
 
 __global__ void k_seg_reset(int* counters) {
     if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0;
@@ -78,6 +84,7 @@ using namespace cs;
 struct cs_segplan {
     int dtype = 0;
     int64_t segments = 0;
+    int64_t base = 0;    // first offset
     int64_t total = 0;
     int64_t max_len = 0;
     int nitems = 0;
@@ -205,6 +212,7 @@ int make_plan_impl(cs_segplan** out, int dtype, const int64_t* h_offsets, int64_
     const int rc0 = seg_build_items(dtype, h_offsets, segments, force_variant, list, items, &p->max_len, p->variant,
                                     p->bucket_segments);
     if (rc0 != CS_OK) { delete p; return rc0; }
+    p->base = segments ? h_offsets[0] : 0;
     p->total = segments ? h_offsets[segments] - h_offsets[0] : 0;
     p->nitems = int(items.size());
     p->shape = seg_shape(dtype, items);
@@ -412,6 +420,31 @@ int cs_segmented_plan_destroy(cs_segplan* plan) {
 
 int cs_segmented_sort(cs_segplan* plan, const void* in, void* out, const int64_t* d_offsets, hipStream_t stream) {
     return run_plan(plan, in, out, d_offsets, stream);
+}
+
+int cs_segmented_sort_pairs(cs_segplan* plan, const void* keys_in, const int32_t* vals_in, void* keys_out,
+                            int32_t* vals_out, const int64_t* d_offsets, hipStream_t stream) {
+    if (!plan) return CS_E_INVALID_VALUE;
+    const int kd = plan->dtype == CS_REC_I32_I32 ? CS_I32 : (plan->dtype == CS_REC_I64_I32 ? CS_I64 : -1);
+    if (kd < 0 || pair_record_dtype(kd) != plan->dtype) return CS_E_INVALID_VALUE;
+    if (plan->total == 0) return CS_OK;
+    if (!keys_in || !vals_in || !keys_out || !vals_out || !d_offsets) return CS_E_INVALID_VALUE;
+    // the records sit at the segments' own positions: [base, base + total)
+    const size_t esz = cs_dtype_size(plan->dtype);
+    const size_t ksz = cs_dtype_size(kd);
+    const int64_t end = plan->base + plan->total;
+    char* rec = nullptr;
+    hipError_t e = pool_malloc_async(reinterpret_cast<void**>(&rec), size_t(end) * esz, stream);
+    if (e != hipSuccess) return int(e);
+    const int64_t b = plan->base;
+    e = pairs_pack(kd, static_cast<const char*>(keys_in) + b * ksz, vals_in + b, rec + b * esz, plan->total, stream);
+    int rc = e == hipSuccess ? run_plan(plan, rec, rec, d_offsets, stream) : int(e);
+    if (rc == CS_OK) {
+        e = pairs_unpack(kd, rec + b * esz, static_cast<char*>(keys_out) + b * ksz, vals_out + b, plan->total, stream);
+        if (e != hipSuccess) rc = int(e);
+    }
+    const hipError_t f = hipFreeAsync(rec, stream);
+    return rc == CS_OK && f != hipSuccess ? int(f) : rc;
 }
 
 int cs_segmented_plan_info(const cs_segplan* plan, int32_t* work_items, int32_t* grid, int64_t* max_len,

@@ -80,6 +80,10 @@ def lib():
         L.cs_segmented_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, vp, i64, u32]
         L.cs_segmented_sort.restype = ctypes.c_int
         L.cs_segmented_sort.argtypes = [vp, vp, vp, vp, vp]
+        L.cs_segmented_sort_pairs.restype = ctypes.c_int
+        L.cs_segmented_sort_pairs.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.cs_sort_pairs.restype = ctypes.c_int
+        L.cs_sort_pairs.argtypes = [ctypes.c_int, vp, vp, i64, vp, vp]
         L.cs_segmented_plan_destroy.restype = ctypes.c_int
         L.cs_segmented_plan_destroy.argtypes = [vp]
         L.cs_segmented_plan_info.restype = ctypes.c_int
@@ -371,6 +375,11 @@ def sort_partition(dtype: int, data: int, n: int, min_parts: int = 0, max_parts:
     check(lib().cs_sort_partition(dtype, data, n, min_parts, max_parts, flags, stream), "cs_sort_partition")
 
 
+def sort_pairs(key_dtype: int, keys: int, vals: int, n: int, ws: "Workspace | None" = None, stream: int = 0) -> None:
+    """cs_sort_pairs: one array of SoA key / int32 payload pairs, in place."""
+    check(lib().cs_sort_pairs(key_dtype, keys, vals, n, ws.handle if ws else None, stream), "cs_sort_pairs")
+
+
 def sort_hybrid(dtype: int, data: int, n: int, ws: "Workspace | None" = None, stream: int = 0) -> None:
     check(lib().cs_sort_hybrid(dtype, data, n, ws.handle if ws else None, stream), "cs_sort_hybrid")
 
@@ -394,6 +403,12 @@ class SegPlan:
 
     def sort(self, src: int, dst: int, d_offsets: int, stream: int = 0) -> None:
         check(lib().cs_segmented_sort(self.handle, src, dst, d_offsets, stream), "cs_segmented_sort")
+
+    def sort_pairs(self, keys_in: int, vals_in: int, keys_out: int, vals_out: int, d_offsets: int,
+                   stream: int = 0) -> None:
+        """cs_segmented_sort_pairs: SoA keys / int32 payloads through a record-type plan."""
+        check(lib().cs_segmented_sort_pairs(self.handle, keys_in, vals_in, keys_out, vals_out, d_offsets, stream),
+              "cs_segmented_sort_pairs")
 
     def info(self) -> dict:
         items, grid = ctypes.c_int32(0), ctypes.c_int32(0)

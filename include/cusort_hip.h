@@ -166,6 +166,16 @@ typedef struct cs_segplan cs_segplan;
 int cs_segmented_plan_create(cs_segplan** plan, int dtype, const int64_t* h_offsets, int64_t segments,
                              uint32_t flags);
 int cs_segmented_sort(cs_segplan* plan, const void* in, void* out, const int64_t* d_offsets, hipStream_t stream);
+/* Key / payload pairs as two arrays (structure of arrays) through a plan
+ * created for the record type (CS_REC_I32_I32 for int32 keys, CS_REC_I64_I32
+ * for int64 keys) and int32 payloads.  No reference counterpart as an API:
+ * the SoA form of the IntAndData record (intanddata.hpp:9-79).  The pairs of
+ * [offsets[0], offsets[segments]) are packed into records in a stream-ordered
+ * temporary, sorted by the plan and unpacked, so the output equals the record
+ * sort of {key[i], payload[i]} split apart.  In place when keys_in ==
+ * keys_out and vals_in == vals_out. */
+int cs_segmented_sort_pairs(cs_segplan* plan, const void* keys_in, const int32_t* vals_in, void* keys_out,
+                            int32_t* vals_out, const int64_t* d_offsets, hipStream_t stream);
 int cs_segmented_plan_destroy(cs_segplan* plan);
 /* Work items, persistent grid size, longest segment, and per bucket b < buckets
  * the chosen variant (0: 32-key register tile, 1: 16-key tile, 2: LDS network)
@@ -257,6 +267,13 @@ int cs_sort_partition(int dtype, void* data, int64_t n, int32_t min_parts, int32
 int cs_sort_partition_ws(int dtype, void* data, int64_t n, int32_t min_parts, int32_t max_parts, uint32_t flags,
                          cs_workspace* ws, hipStream_t stream);
 int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream_t stream);
+/* One array of n key / payload pairs as two arrays (key_dtype CS_I32 or
+ * CS_I64, int32 payloads), sorted in place by key through the hybrid sort of
+ * the matching record type (no reference counterpart as an API: the SoA form
+ * of IntAndData, intanddata.hpp:9-79).  The pairs are packed into records in
+ * a stream-ordered temporary, sorted (ws: NULL or a workspace of the record
+ * type) and unpacked: the output equals the record sort split apart. */
+int cs_sort_pairs(int key_dtype, void* keys, int32_t* vals, int64_t n, cs_workspace* ws, hipStream_t stream);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under stream capture, serving loops, harnesses that forbid
  * it).  The in-place merge sort of cs_sort_merge, with its scratch (n
