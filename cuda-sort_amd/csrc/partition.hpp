@@ -31,7 +31,7 @@
 
 namespace cs {
 
-constexpr int kPartThreads = 256;
+constexpr int kPartThreads = 512;        // threads of the count kernel
 constexpr int kPartMaxSplitters = 2047;          // 2P-1 <= 4095 buckets (12-bit bucket ids)
 
 __device__ __forceinline__ uint32_t part_hash(uint32_t x) {   // PCG output permutation of x
@@ -53,7 +53,6 @@ __global__ void k_part_sample(const typename Elem<DT>::Storage* src, int64_t n, 
 
 // Elements per thread per tile of the count and scatter kernels.
 template <int DT> constexpr int part_items() { return sizeof(typename Elem<DT>::Storage) == 4 ? 16 : 8; }
-template <int DT> constexpr int part_tile() { return part_items<DT>() * kPartThreads; }
 // The scatter stages larger tiles (16384 4-byte / 8192 8-byte / 4096 16-byte
 // elements) with a wider workgroup: each bucket's run per tile is then long
 // enough that its writes fill whole cache lines.
@@ -116,13 +115,13 @@ __global__ void k_part_prep(const typename Elem<DT>::Storage* sorted, int m, int
 
 // Classification.  A lookup table indexed by kPartLutBits bits of the key's
 // order-preserving unsigned image, taken relative to the segment's lower
-// bound and scaled to its width, narrows the splitter search:
-// lut[x] = number of splitters whose prefix is below x, so the splitters
-// that can tie with or bracket a key of prefix x are sp[lut[x] .. lut[x+1]).
-// Any prefix that does not decrease as the key grows keeps this exact.  For
-// spread-out keys the range holds zero to a few splitters; for keys crowded
-// under one prefix it is the full list and the search is the plain binary
-// search.  Built by every workgroup from the splitters in LDS.
+// bound and scaled to its width, narrows the splitter search to the
+// splitters that share the key's prefix (any prefix that does not decrease
+// as the key grows keeps this exact).  With 4096 entries and a few hundred
+// splitters per segment, spread-out keys meet zero or one candidate; for
+// keys crowded under one prefix the range is the full list and the search
+// is the plain binary search.  Built by every workgroup from the splitters
+// in LDS.
 constexpr int kPartLutBits = 12;
 constexpr int kPartLutSize = (1 << kPartLutBits) + 1;
 
@@ -142,7 +141,7 @@ struct PartLutMap {   // prefix = min((image - base) >> shift, 2^kPartLutBits - 
 template <class K> __device__ __forceinline__ uint32_t part_prefix(K k, PartLutMap m) {
     if constexpr (sizeof(K) == 4) {   // 32-bit arithmetic: base and shift fit
         const uint32_t u = uint32_t(part_image(k)), b = uint32_t(m.base);
-        const uint32_t r = u > b ? (u - b) >> m.shift : 0u;
+        const uint32_t r = __builtin_elementwise_sub_sat(u, b) >> m.shift;   // 0 below the base
         return r < uint32_t(kPartLutSize - 1) ? r : uint32_t(kPartLutSize - 2);
     } else {
         const uint64_t u = part_image(k);
@@ -162,45 +161,103 @@ template <class K> __device__ __forceinline__ PartLutMap part_lut_map(const K* s
     return PartLutMap{base, bits > kPartLutBits ? bits - kPartLutBits : 0};
 }
 
-// lut[x] packs the splitter range of prefix x: low half = number of
-// splitters whose prefix is below x, high half = number whose prefix is at
-// most x (one LDS read per key).  sp must hold two readable entries past ns.
-template <class K>
-__device__ __forceinline__ void part_build_lut(const K* sp, int ns, uint32_t* lut, PartLutMap m) {
-    for (int x = int(threadIdx.x); x < kPartLutSize - 1; x += int(blockDim.x)) {
-        int lo = 0, hi = ns;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (part_prefix(sp[mid], m) < uint32_t(x)) lo = mid + 1;
-            else hi = mid;
-        }
-        int end = lo;
-        while (end < ns && part_prefix(sp[end], m) == uint32_t(x)) ++end;
-        lut[x] = uint32_t(lo) | (uint32_t(end) << 16);
+// Lookup-table entry of prefix x: range = lo | hi << 16, where lo splitters
+// have a prefix below x and hi have a prefix at most x, so the splitters that
+// can tie with or bracket a key of prefix x are sp[lo .. hi).  For 4-byte
+// keys the entry also carries sp[lo]: a key with at most one candidate
+// splitter (almost every key once the table is finer than the splitters) is
+// then classified by one 8-byte LDS read and no other.
+template <class K, bool kCarry = (sizeof(K) == 4)> struct alignas(8) PartLutEntry {
+    uint32_t range;
+    K first;
+};
+template <class K> struct PartLutEntry<K, false> {
+    uint32_t range;
+};
+
+// Builds the table from the splitters sp[0, ns) (sorted, two readable
+// entries past ns) with the workgroup's NT threads: a histogram of the
+// splitters' prefixes, then one exclusive scan of it (lo(x) = sum of the
+// counts below x, hi(x) = lo(x) + count(x)).  wsum holds NT / 64 ints.
+template <int NT, class K>
+__device__ __forceinline__ void part_build_lut(const K* sp, int ns, PartLutEntry<K>* lut, PartLutMap m, int* wsum) {
+    constexpr int X = kPartLutSize - 1;          // entries
+    constexpr int C = X / NT;                    // entries per thread, contiguous
+    static_assert(X % NT == 0 && NT % 64 == 0, "table entries split evenly over the workgroup");
+    const int tid = int(threadIdx.x);
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int x = tid; x < X; x += NT) lut[x].range = 0;
+    __syncthreads();
+    for (int i = tid; i < ns; i += NT) atomicAdd(&lut[part_prefix(sp[i], m)].range, 1u);
+    __syncthreads();
+    int c[C];
+    int sum = 0;
+    #pragma unroll
+    for (int q = 0; q < C; ++q) {
+        c[q] = int(lut[tid * C + q].range);
+        sum += c[q];
+    }
+    int incl = sum;
+    #pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int lo = incl - sum;
+    for (int q = 0; q < wave; ++q) lo += wsum[q];
+    #pragma unroll
+    for (int q = 0; q < C; ++q) {
+        PartLutEntry<K> e;
+        e.range = uint32_t(lo) | (uint32_t(lo + c[q]) << 16);
+        if constexpr (sizeof(K) == 4) e.first = sp[lo];   // lo <= ns: readable
+        lut[tid * C + q] = e;
+        lo += c[q];
     }
 }
 
 // Bucket of key k: j = number of splitters <= k; a key equal to splitter
-// j-1 goes to the equality bucket 2j-1, any other key to bucket 2j.  Up to
-// two candidate splitters (spread-out keys) are compared without a branch;
-// a longer candidate range (keys crowded under one prefix) is searched.
+// j-1 goes to the equality bucket 2j-1, any other key to bucket 2j.  The
+// splitters below the key's candidate range are below it and those above
+// are above it (the prefix does not decrease as the key grows), and a key
+// left of every candidate cannot equal splitter lo-1, whose prefix is
+// smaller than its own.  One candidate is compared in registers (4-byte
+// keys: taken from the table entry); a longer candidate range (keys crowded
+// under one prefix) is binary-searched.
 template <class E>
-__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp, const uint32_t* lut,
-                                           PartLutMap m) {
-    const uint32_t r = lut[part_prefix(k, m)];
-    int lo = int(r & 0xFFFFu), hi = int(r >> 16);
-    if (hi - lo > 2) {
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (!(k < sp[mid])) lo = mid + 1;
-            else hi = mid;
-        }
-    } else {   // sp[lo], sp[lo + 1] are readable (two entries past ns)
-        const typename E::Key s0 = sp[lo], s1 = sp[lo + 1];
-        lo += int(lo < hi && !(k < s0)) + int(lo + 1 < hi && !(k < s1));
+__device__ __forceinline__ int part_bucket(typename E::Key k, const typename E::Key* sp,
+                                           const PartLutEntry<typename E::Key>* lut, PartLutMap m) {
+    using K = typename E::Key;
+    const uint32_t x = part_prefix(k, m);
+    uint32_t range;
+    K s{};
+    if constexpr (sizeof(K) == 4) {   // range and first candidate in one 8-byte read
+        const uint2 raw = reinterpret_cast<const uint2*>(lut)[x];
+        range = raw.x;
+        __builtin_memcpy(&s, &raw.y, 4);
+    } else {
+        range = lut[x].range;
     }
-    const typename E::Key below = sp[lo > 0 ? lo - 1 : 0];   // read unconditionally: no branch
-    return (lo > 0 && !(below < k)) ? 2 * lo - 1 : 2 * lo;
+    const int lo = int(range & 0xFFFFu), hi = int(range >> 16);
+    if constexpr (sizeof(K) != 4) {
+        if (hi - lo == 1) s = sp[lo];
+    }
+    // zero or one candidate, without a branch
+    const bool ge = hi > lo && !(k < s);
+    int j = lo + int(ge);
+    bool eq = ge && s == k;
+    if (hi - lo > 1) {   // several candidates: binary search
+        j = lo;
+        int b = hi;
+        while (j < b) {
+            const int mid = (j + b) >> 1;
+            if (!(k < sp[mid])) j = mid + 1;
+            else b = mid;
+        }
+        eq = j > lo && sp[j - 1] == k;
+    }
+    return 2 * j - int(eq);
 }
 
 // Adds `one` (1 per valid lane) to counter[b] for every valid lane and
@@ -225,6 +282,33 @@ __device__ __forceinline__ int part_wave_add(int* counter, int b, bool valid) {
     return valid ? atomicAdd(&counter[b], 1) : 0;
 }
 
+// Raw buffer loads of a workgroup's element range: a position at or past
+// `elems` reads as zero instead of leaving the range, so a tile's loads need
+// no clamping, and their addresses are 32-bit offsets from a scalar base
+// (a range is at most kPartBlockElems elements).
+template <class S>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t part_rsrc(const S* base, int elems) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<S*>(base), short(0), (elems > 0 ? elems : 0) * int(sizeof(S)),
+                                             0x00020000);   // raw (untyped, unswizzled) buffer
+}
+
+template <class S> __device__ __forceinline__ S part_load(__amdgpu_buffer_rsrc_t r, int i) {
+    const int off = i * int(sizeof(S));
+    S s;
+    if constexpr (sizeof(S) == 4) {
+        const int v = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+        __builtin_memcpy(&s, &v, 4);
+    } else if constexpr (sizeof(S) == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        __builtin_memcpy(&s, &v, 8);
+    } else {
+        static_assert(sizeof(S) == 16, "4-, 8- or 16-byte elements");
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        __builtin_memcpy(&s, &v, 16);
+    }
+    return s;
+}
+
 // Per-workgroup histogram: every thread classifies part_items keys per tile,
 // their loads all in flight together (coalesced across the workgroup).
 // Dynamic LDS of k_part_count for up to ns splitters per workgroup:
@@ -238,72 +322,73 @@ __global__ void __launch_bounds__(kPartThreads)
 k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key* splitters, const PartBlock* work,
              int max_ns, int64_t* counts, const int64_t* nwork) {
     using E = Elem<DT>;
+    using S = typename E::Storage;
     if (nwork && int64_t(blockIdx.x) >= *nwork) return;   // grid sized by a bound, work list built on the device
     constexpr int IT = part_items<DT>();
+    constexpr int NT = kPartThreads;
     extern __shared__ __align__(16) unsigned char part_lds[];
     typename E::Key* sp = reinterpret_cast<typename E::Key*>(part_lds);
     int* hist = reinterpret_cast<int*>(part_lds + ((size_t(max_ns + 2) * sizeof(typename E::Key) + 15) & ~size_t(15)));
-    __shared__ uint32_t lut[kPartLutSize - 1];
+    __shared__ PartLutEntry<typename E::Key> lut[kPartLutSize - 1];
+    __shared__ int wsum[NT / 64];
     __shared__ int crowded;   // some bucket drew a quarter of the first tile
     const PartBlock w = work[blockIdx.x];
     const int ns = w.ns;
     const int nb = 2 * ns + 1;
     const int tid = int(threadIdx.x);
     if (tid == 0) crowded = 0;
-    for (int i = tid; i < ns + 2; i += kPartThreads) sp[i] = i < ns ? splitters[w.sp0 + i] : typename E::Key{};
-    for (int i = tid; i < nb; i += kPartThreads) hist[i] = 0;
+    for (int i = tid; i < ns + 2; i += NT) sp[i] = i < ns ? splitters[w.sp0 + i] : typename E::Key{};
+    for (int i = tid; i < nb; i += NT) hist[i] = 0;
     const PartLutMap map = part_lut_map(splitters, w.lo, w.hi);
     __syncthreads();
-    part_build_lut(sp, ns, lut, map);
+    part_build_lut<NT>(sp, ns, lut, map, wsum);
     __syncthreads();
     // 32-bit element indices relative to the workgroup's range (at most
     // kPartBlockElems long).  The next tile's loads are issued before this
-    // tile is classified; loads past the end re-read the last element
-    // (unconditional loads keep the prefetched tile in flight; their results
-    // are masked below).
-    const typename E::Storage* base = src + w.begin;
+    // tile is classified; loads past the end read zero (their results are
+    // masked below).
+    const S* base = src + w.begin;
     const int len = int(w.end - w.begin);
-    const int last = len - 1;
-    constexpr int STEP = IT * kPartThreads;
+    constexpr int STEP = IT * NT;
     bool agg = false;
-    typename E::Storage cur[IT];
-    #pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const int i = k * kPartThreads + tid;
-        cur[k] = base[i < len ? i : last];
+    S cur[IT];
+    {
+        const __amdgpu_buffer_rsrc_t r = part_rsrc(base, len);
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) cur[k] = part_load<S>(r, k * NT + tid);
     }
     for (int t0 = 0; t0 < len; t0 += STEP) {
-        typename E::Storage nxt[IT];
+        S nxt[IT];
+        const __amdgpu_buffer_rsrc_t r = part_rsrc(base + t0 + STEP, len - t0 - STEP);
         #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-            const int i = t0 + STEP + k * kPartThreads + tid;
-            nxt[k] = base[i < len ? i : last];
-        }
+        for (int k = 0; k < IT; ++k) nxt[k] = part_load<S>(r, k * NT + tid);
         if (agg) {   // workgroup-uniform: sorted or repeated keys crowd one counter
             #pragma unroll
             for (int k = 0; k < IT; ++k) {
-                const bool valid = t0 + k * kPartThreads + tid < len;
+                const bool valid = t0 + k * NT + tid < len;
                 const int b = valid ? part_bucket<E>(E::key(cur[k]), sp, lut, map) : 0;
                 (void)part_wave_add(hist, b, valid);
             }
+        } else if (t0 + STEP <= len) {   // a whole tile: no lane is masked
+            #pragma unroll
+            for (int k = 0; k < IT; ++k) atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
         } else {
             #pragma unroll
             for (int k = 0; k < IT; ++k)
-                if (t0 + k * kPartThreads + tid < len)
-                    atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
+                if (t0 + k * NT + tid < len) atomicAdd(&hist[part_bucket<E>(E::key(cur[k]), sp, lut, map)], 1);
         }
         #pragma unroll
         for (int k = 0; k < IT; ++k) cur[k] = nxt[k];
         if (t0 == 0) {   // after the first tile: switch to wave-aggregated counting if crowded
             __syncthreads();
-            for (int b = tid; b < nb; b += kPartThreads)
+            for (int b = tid; b < nb; b += NT)
                 if (hist[b] >= STEP / 4) crowded = 1;
             __syncthreads();
             agg = crowded != 0;
         }
     }
     __syncthreads();
-    for (int b = tid; b < nb; b += kPartThreads) counts[w.cell + int64_t(b) * w.stride] = hist[b];
+    for (int b = tid; b < nb; b += NT) counts[w.cell + int64_t(b) * w.stride] = hist[b];
 }
 
 // Dynamic LDS of k_part_scatter for up to ns splitters per workgroup:
@@ -354,7 +439,7 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
     uint16_t* sbkt = reinterpret_cast<uint16_t*>(reinterpret_cast<unsigned char*>(stage) +
                                                  part_align(size_t(TILE) * sizeof(S)));
     __shared__ int wsum[NT / 64];
-    __shared__ uint32_t lut[kPartLutSize - 1];
+    __shared__ PartLutEntry<K> lut[kPartLutSize - 1];
     __shared__ int crowded;   // a bucket drew a quarter of some tile: rank with wave-aggregated atomics
     const int tid = int(threadIdx.x);
     const int lane = tid & 63, wave = tid >> 6;
@@ -366,29 +451,26 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
     }
     const PartLutMap map = part_lut_map(splitters, w.lo, w.hi);
     __syncthreads();
-    part_build_lut(sp, ns, lut, map);
+    part_build_lut<NT>(sp, ns, lut, map, wsum);
     __syncthreads();
     const int per_thread = (nb + NT - 1) / NT;   // scan chunk of each thread
     // 32-bit indices relative to the workgroup's range; the next tile's loads
-    // are issued before this tile is processed (loads past the end re-read
-    // the last element, as in the count kernel)
+    // are issued before this tile is processed (loads past the end read zero,
+    // as in the count kernel)
     const S* base = src + w.begin;
     const int len = int(w.end - w.begin);
-    const int last = len - 1;
     S elem[IT];
-    #pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const int i = k * NT + tid;
-        elem[k] = base[i < len ? i : last];
+    {
+        const __amdgpu_buffer_rsrc_t r = part_rsrc(base, len);
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) elem[k] = part_load<S>(r, k * NT + tid);
     }
     for (int t0 = 0; t0 < len; t0 += TILE) {
         const int tn = len - t0 < TILE ? len - t0 : TILE;
         S nxt[IT];
+        const __amdgpu_buffer_rsrc_t r = part_rsrc(base + t0 + TILE, len - t0 - TILE);
         #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-            const int i = t0 + TILE + k * NT + tid;
-            nxt[k] = base[i < len ? i : last];
-        }
+        for (int k = 0; k < IT; ++k) nxt[k] = part_load<S>(r, k * NT + tid);
         uint32_t br[IT];   // bucket (12 bits) | rank inside the tile << 12
         if (crowded) {   // workgroup-uniform (set before the last barrier)
             #pragma unroll
@@ -397,6 +479,12 @@ k_part_scatter(const typename Elem<DT>::Storage* src, const typename Elem<DT>::K
                 const int b = x < tn ? part_bucket<E>(E::key(elem[k]), sp, lut, map) : 0;
                 const int r = part_wave_add(toff, b, x < tn);
                 br[k] = uint32_t(b) | (uint32_t(r) << 12);
+            }
+        } else if (tn == TILE) {   // a whole tile: no lane is masked
+            #pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int b = part_bucket<E>(E::key(elem[k]), sp, lut, map);
+                br[k] = uint32_t(b) | (uint32_t(atomicAdd(&toff[b], 1)) << 12);
             }
         } else {
             #pragma unroll
