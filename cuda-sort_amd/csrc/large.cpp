@@ -389,15 +389,15 @@ int cs_sort_hybrid(int dtype, void* data, int64_t n, cs_workspace* ws, hipStream
 
 int64_t cs_hybrid_threshold(int dtype) {
     // Crossover measured on MI355X (uniform keys; merge / partition ms):
-    //   int32                2^24 0.56 / 0.83   2^26 2.54 / 1.70   2^28 10.3 / 5.23
-    //   uint64               2^22 0.39 / 0.87   2^24 1.23 / 1.15   2^26 5.35 / 2.78
-    //   int32+int32 records  2^24 0.95 / 1.00   2^25 2.16 / 1.44
-    //   int64+int32 records  2^21 0.59 / 0.80   2^22 1.09 / 0.92
+    //   int32                2^22 0.30 / 0.32   2^23 0.36 / 0.35   2^24 0.59 / 0.46
+    //   uint64               2^22 0.37 / 0.49   2^23 0.61 / 0.55   2^24 1.08 / 0.77
+    //   int32+int32 records  2^22 0.31 / 0.40   2^23 0.52 / 0.47   2^24 0.93 / 0.66
+    //   int64+int32 records  2^20 0.34 / 0.44   2^21 0.60 / 0.52   2^22 1.10 / 0.62
     // The reference encodes its own crossover as CuSortTuning
     // (cusort.hpp:2476-2483: never for 4-byte keys, 2^27 for 8-byte ones).
     if (!valid_dtype(dtype)) return 0;
     const size_t size = cs_dtype_size(dtype);
-    return int64_t(1) << (size >= 16 ? 22 : (size >= 8 ? 24 : 25));
+    return int64_t(1) << (size >= 16 ? 21 : 23);
 }
 
 }  // extern "C"

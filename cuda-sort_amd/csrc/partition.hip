@@ -339,28 +339,34 @@ PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
     const LargeKernels lk = large_kernels_for(dtype);
     const size_t esz = cs_dtype_size(dtype);
     PartShape sh;
-    // Buckets average 3/8 of the tile (3072 4-byte / 1536 wider keys): with
-    // 16-64 samples per part the largest of ~10^5 buckets stays under the
-    // tile and most fill the next power of two to ~75 %.
-    sh.target = lk.chunk * 3 / 8;
+    // Buckets average 1536 elements, 3/4 of a 2048-key tile: with 12-64
+    // samples per part most fill that power of two to ~75 %, and the largest
+    // of ~10^5 buckets stays far under the chunk (8192 4-byte / 4096 wider
+    // elements) above which a bucket takes the long path.  2^28 int32 (ms
+    // per sort, 16 samples per part): 1400 3.80, 1536 3.76, 1700 3.91,
+    // 3072 4.13; the 2048-key tile sorts a key in fewer layers than the
+    // 4096-key one, which outweighs twice the parts.
+    sh.target = 1536;
     int64_t P = (n + sh.target - 1) / sh.target;
     if (min_parts > 0) P = std::max<int64_t>(P, min_parts);
     if (max_parts > 0) P = std::min<int64_t>(P, max_parts);
-    const int64_t pmax = int64_t(kPartMaxSplitters + 1) * (kPartMaxSplitters + 1);
+    const int maxp = part_kernels_for(dtype).max_splitters + 1;   // parts per level
+    const int64_t pmax = int64_t(maxp) * maxp;
     P = std::max<int64_t>(2, std::min(P, pmax));
-    sh.p1 = int(P);
+    sh.p1 = int(std::min<int64_t>(P, maxp));
     if (P > kPartOneLevel) {
         sh.p1 = int(std::ceil(std::sqrt(double(P))));
         sh.p2 = int(P / sh.p1);   // p1 * p2 <= P: a caller's part bound holds
-        sh.p1 = std::min(sh.p1, kPartMaxSplitters + 1);
-        sh.p2 = std::min(sh.p2, kPartMaxSplitters + 1);
+        sh.p1 = std::min(sh.p1, maxp);
+        sh.p2 = std::min(sh.p2, maxp);
     }
     sh.parts = int64_t(sh.p1) * sh.p2;
-    // Samples per part.  Two levels, 2^28 int32: 16 per part 4.97 ms, 32 per
-    // part 5.09 ms (a smaller sample to sort outweighs a wider spread of
-    // bucket sizes); 2^27 uint64: 5.20 ms against 4.92 (its buckets of 1536
-    // spill into the next tile size sooner), so wider keys keep 32.
-    sh.m = sh.parts * (sh.p2 > 1 ? (esz == 4 ? 16 : 32) : 64);
+    // Samples per part, two levels.  2^28 int32 (ms per sort): 8 per part
+    // 3.77, 12 3.75, 16 3.76, 24 3.78 (a smaller sample to sort against a
+    // wider spread of bucket sizes); 2^27 uint64: 16 4.42, 32 4.25, 48 4.23
+    // (8-byte keys make the bucket sort dearer per spilled key), so wider
+    // elements keep 32.
+    sh.m = sh.parts * (sh.p2 > 1 ? (esz == 4 ? 12 : 32) : 64);
     return sh;
 }
 

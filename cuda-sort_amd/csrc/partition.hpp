@@ -396,12 +396,24 @@ k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key
 // positions, the tile staged in bucket order and the bucket of every staged
 // element.
 constexpr size_t part_align(size_t x) { return (x + 15) & ~size_t(15); }
-template <int DT> size_t part_scatter_lds(int ns) {
+template <int DT> constexpr size_t part_scatter_lds(int ns) {
     using E = Elem<DT>;
     const size_t nb = size_t(2 * ns + 1);
     return part_align(size_t(ns + 2) * sizeof(typename E::Key)) + part_align(nb * 4) + 2 * part_align(nb * 8) +
            part_align(size_t(part_scatter_tile<DT>()) * sizeof(typename E::Storage)) +
            part_align(size_t(part_scatter_tile<DT>()) * 2) + 64;
+}
+
+// Most splitters per workgroup for which the scatter fits a CU's 160 KB of
+// LDS: its dynamic carve-up, the lookup table and a margin for the small
+// static arrays.  Both levels' part counts are capped by it (past it, the
+// buckets grow and the rare one longer than a tile takes the long path).
+template <int DT> constexpr int part_max_splitters() {
+    constexpr size_t kLds = 160 * 1024;
+    constexpr size_t fixed = sizeof(PartLutEntry<typename Elem<DT>::Key>) * (kPartLutSize - 1) + 256;
+    int ns = kPartMaxSplitters;
+    while (ns > 1 && part_scatter_lds<DT>(ns) + fixed > kLds) --ns;
+    return ns;
 }
 
 // Scatter.  Per tile of part_scatter_tile elements: classify, rank inside
@@ -714,6 +726,7 @@ struct PartKernels {
     hipError_t (*seg_merge)(const void* src, void* dst, MergeTile* items, int nitems, int64_t run,
                             int copy_only, hipStream_t s) = nullptr;
     int key_size = 0;
+    int max_splitters = 0;   // part_max_splitters: splitters per workgroup the scatter's LDS holds
 };
 
 template <int DT>
@@ -723,6 +736,7 @@ PartKernels part_kernels() {
     using K = typename E::Key;
     PartKernels k;
     k.key_size = int(sizeof(K));
+    k.max_splitters = part_max_splitters<DT>();
     k.sample = [](const void* src, int64_t n, void* samples, int m, hipStream_t s) {
         hipLaunchKernelGGL(k_part_sample<DT>, dim3(unsigned((m + 255) / 256)), dim3(256), 0, s,
                            static_cast<const S*>(src), n, static_cast<S*>(samples), m);

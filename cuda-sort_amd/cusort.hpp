@@ -212,13 +212,13 @@ public:
 // CuSortTuning<T> (reference cusort.hpp:2476-2483): the element count from
 // which HybridSort / HybridSortV3 take the partitioned path, a specialisable
 // compile-time constant.  The default is the crossover measured on MI355X
-// (merge below, partition above: 2^25 for 4-byte elements, 2^24 for 8-byte
-// ones, 2^22 for 16-byte records), where the reference's default (tuned on
-// its own GPUs) never partitions 4-byte keys and partitions 8-byte ones from
-// 2^27.  A caller's specialisation is what HybridSort consults.
+// (merge below, partition above: 2^23 for 4- and 8-byte elements, 2^21 for
+// 16-byte records), where the reference's default (tuned on its own GPUs)
+// never partitions 4-byte keys and partitions 8-byte ones from 2^27.  A
+// caller's specialisation is what HybridSort consults.
 template <typename T>
 struct CuSortTuning {
-    static constexpr int PartitionThreshold = sizeof(T) >= 16 ? (1 << 22) : sizeof(T) >= 8 ? (1 << 24) : (1 << 25);
+    static constexpr int PartitionThreshold = sizeof(T) >= 16 ? (1 << 21) : (1 << 23);
 };
 
 // ------------------------------------------------------- large arrays ----

@@ -89,10 +89,10 @@ def test_pairs_arguments(cs):
 
 
 def test_hybrid_thresholds(cs):
-    # measured merge / partition crossovers: 2^25 for 4-byte, 2^24 for 8-byte,
-    # 2^22 for 16-byte elements; 0 for an unknown element type
-    want = {cs.I32: 25, cs.U32: 25, cs.F32: 25, cs.U64: 24, cs.I64: 24, cs.F64: 24, cs.REC_I32_I32: 24,
-            cs.REC_I64_I32: 22}
+    # measured merge / partition crossovers: 2^23 for 4- and 8-byte, 2^21 for
+    # 16-byte elements; 0 for an unknown element type
+    want = {cs.I32: 23, cs.U32: 23, cs.F32: 23, cs.U64: 23, cs.I64: 23, cs.F64: 23, cs.REC_I32_I32: 23,
+            cs.REC_I64_I32: 21}
     for dt, lg in want.items():
         assert cs.hybrid_threshold(dt) == 1 << lg, dt
     assert cs.hybrid_threshold(99) == 0
