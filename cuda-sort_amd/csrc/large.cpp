@@ -147,10 +147,15 @@ int cu_count() {
     return cus > 0 ? cus : 1;
 }
 
-// Arena bytes for sorts of up to `capacity` elements through either path.
+// Arena bytes for sorts of up to `capacity` elements through either path,
+// the partition path with its default parts or any smaller part bound: a
+// bound of at most 256 parts takes one level with 64 samples per part and
+// can need more scratch than the default two levels (both shapes are
+// covered; fewer parts on two levels need less).
 size_t arena_bound(int dtype, size_t capacity) {
-    return std::max(merge_scratch_bytes(dtype, int64_t(capacity)),
-                    partition_scratch_bytes(dtype, int64_t(capacity), 0, 0));
+    return std::max({merge_scratch_bytes(dtype, int64_t(capacity)),
+                     partition_scratch_bytes(dtype, int64_t(capacity), 0, 0),
+                     partition_scratch_bytes(dtype, int64_t(capacity), 0, 256)});
 }
 
 // Brackets one sort on `stream` that uses the workspace.

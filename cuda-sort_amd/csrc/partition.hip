@@ -177,15 +177,22 @@ k_part_level2(const int64_t* counts1, int64_t blocks1, int nb1, int64_t n, int p
 // work items cover `spi` consecutive list entries of one class, as the host
 // planner lays them out (the order inside a class is free: every segment is
 // sorted on its own).  Equality buckets (one key value each) are left as
-// they are when the sort is in place.  A longer bucket than the tile gets
-// class kPlanLong and sends the call to the host planner, which cuts it
-// into chunks.
+// they are when the sort is in place.  A longer bucket than the chunk gets
+// class kPlanLong (chunk sorts and merge passes).  A bucket a little longer
+// than the single-wave tile, 2^split < len <= 2^(split+1), is listed twice,
+// as its head in class `split` and its tail in the tail's own class, and
+// goes to the split list, whose buckets k_part_split_merge merges after the
+// persistent sort: two single-wave tiles and a merge cost less than one
+// padded tile over several waves (cls = kPlanSplitCls + tail class).
 constexpr int kPlanClasses = 16;
 constexpr int kPlanLong = 14;
+constexpr int kPlanSplit = 15;       // ccount slot: split buckets
+constexpr int kPlanSplitCls = 16;
 
 struct PlanTable {
     int maxb;
     int small_max;
+    int split;   // log2 of the head of a split bucket; -1: no splitting
     int spi[kPlanClasses];
     int variant[kPlanClasses];
 };
@@ -195,27 +202,84 @@ struct PlanTable {
 constexpr int kPlanMeta = 4 * kPlanClasses + 4;
 
 
-__global__ void k_plan_classify(int64_t* starts, const unsigned char* equal, int skip_equal, int64_t nb,
-                                const int64_t* d_nb, int64_t n, int64_t chunk, unsigned char* cls, int* ccount,
-                                int* long_list) {
+// Adds one to h[c] for every lane of the wave with 0 <= c < kPlanClasses,
+// c != kPlanLong, and
+// returns the lane's previous count (its slot among the class's entries):
+// one LDS atomic per distinct class of the wave (the buckets of a level
+// mostly share one or two classes; per-lane atomics on one address
+// serialise).  Every lane of the wave calls it.
+__device__ __forceinline__ int plan_count(int* h, int c) {
+    const int lane = int(__lane_id());
+    uint64_t todo = __ballot(c >= 0 && c < kPlanClasses && c != kPlanLong);
+    int slot = 0;
+    while (todo) {
+        const int first = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+        const int cf = __builtin_amdgcn_readlane(c, first);
+        const uint64_t same = __ballot(c == cf) & todo;
+        int base = 0;
+        if (lane == first) base = atomicAdd(&h[cf], __popcll(same));
+        base = __builtin_amdgcn_readlane(base, first);
+        if (c == cf) slot = base + __popcll(same & ((uint64_t(1) << lane) - 1));
+        todo &= ~same;
+    }
+    return slot;
+}
+
+// Class of final bucket i: c1 = the class it is listed in (a split bucket:
+// its head's), c2 = a split bucket's tail class (else -1); cls[i] as above.
+struct PlanClass {
+    int cls, c1, c2;
+};
+
+__device__ __forceinline__ PlanClass plan_class(int64_t len, bool eq, int64_t chunk, int split) {
+    const int64_t head = split >= 0 ? int64_t(1) << split : 0;
+    PlanClass p{255, -1, -1};   // empty, or an equality bucket sorted in place: nothing to do
+    if (eq) return p;
+    if (len > chunk) {
+        p.cls = kPlanLong;
+    } else if (split >= 0 && len > head && len <= 2 * head) {
+        const int64_t tl = len - head;
+        p.c2 = tl > 1 ? 64 - __clzll(uint64_t(tl - 1)) : 0;
+        p.c1 = split;
+        p.cls = kPlanSplitCls + p.c2;
+    } else if (len > 0) {
+        p.cls = len > 1 ? 64 - __clzll(uint64_t(len - 1)) : 0;
+        p.c1 = p.cls;
+    }
+    return p;
+}
+
+// Few workgroups, each over a stride of buckets, so that every class count
+// reaches device memory with one atomic per workgroup (same-address device
+// atomics serialise: one per bucket workgroup of 256 cost ~35 us at 2^28).
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanGrid = 64;
+
+__global__ void __launch_bounds__(kPlanThreads)
+k_plan_classify(int64_t* starts, const unsigned char* equal, int skip_equal, int64_t nb, const int64_t* d_nb,
+                int64_t n, int64_t chunk, int split, unsigned char* cls, int* ccount, int* long_list) {
     __shared__ int h[kPlanClasses];
     if (threadIdx.x < kPlanClasses) h[threadIdx.x] = 0;
     __syncthreads();
     if (d_nb) nb = *d_nb;   // bucket count of a level planned on the device
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < nb) {
-        const int64_t len = (i + 1 < nb ? starts[i + 1] : n) - starts[i];
-        int c = 255;   // empty, or an equality bucket sorted in place: nothing to do
-        if (skip_equal && equal[i]) c = 255;
-        else if (len > chunk) c = kPlanLong;
-        else if (len > 0) c = len > 1 ? 64 - __clzll(uint64_t(len - 1)) : 0;
-        cls[i] = static_cast<unsigned char>(c);
-        if (c < kPlanLong) atomicAdd(&h[c], 1);
-        if (c == kPlanLong) long_list[atomicAdd(&ccount[kPlanLong], 1)] = int(i);   // rare
+    const int64_t stride = int64_t(gridDim.x) * kPlanThreads;
+    for (int64_t i0 = int64_t(blockIdx.x) * kPlanThreads; i0 < nb; i0 += stride) {   // uniform trip count
+        const int64_t i = i0 + threadIdx.x;
+        PlanClass p{255, -1, -1};
+        if (i < nb) {
+            const int64_t len = (i + 1 < nb ? starts[i + 1] : n) - starts[i];
+            p = plan_class(len, skip_equal && equal[i], chunk, split);
+            cls[i] = static_cast<unsigned char>(p.cls);
+            if (p.cls == kPlanLong) long_list[atomicAdd(&ccount[kPlanLong], 1)] = int(i);   // rare
+        }
+        plan_count(h, p.c1);
+        plan_count(h, p.c2 >= 0 ? kPlanSplit : -1);   // split buckets, counted in slot kPlanSplit
+        plan_count(h, p.c2);
     }
-    if (i == 0) starts[nb] = n;   // closing offset of the segment CSR
+    if (blockIdx.x == 0 && threadIdx.x == 0) starts[nb] = n;   // closing offset of the segment CSR
     __syncthreads();
-    if (threadIdx.x < kPlanLong && h[threadIdx.x]) atomicAdd(&ccount[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x < kPlanClasses && threadIdx.x != kPlanLong && h[threadIdx.x])
+        atomicAdd(&ccount[threadIdx.x], h[threadIdx.x]);
 }
 
 // Layout of the class-grouped list and the item ranges, classes from maxb
@@ -243,8 +307,12 @@ __device__ void plan_layout(const int* ccount, const PlanTable& t, PlanLayout* L
     L->items = items;
 }
 
-__global__ void k_plan_fill(const unsigned char* cls, int64_t nb, const int64_t* d_nb, int* meta, PlanTable t,
-                            int* list) {
+// Two passes over the workgroup's stride of buckets: the class counts (one
+// device atomic per class for the workgroup's places in the list and in
+// the split list), then the entries, in the same order.
+__global__ void __launch_bounds__(kPlanThreads)
+k_plan_fill(const unsigned char* cls, int64_t nb, const int64_t* d_nb, int* meta, PlanTable t, int* list,
+            int* split_list) {
     __shared__ int h[kPlanClasses], base[kPlanClasses];
     __shared__ PlanLayout L;
     int* cursor = meta + kPlanClasses;
@@ -252,13 +320,39 @@ __global__ void k_plan_fill(const unsigned char* cls, int64_t nb, const int64_t*
     if (threadIdx.x == 0) plan_layout(meta, t, &L);
     if (d_nb) nb = *d_nb;
     __syncthreads();
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int c = i < nb ? int(cls[i]) : 255;
-    const int local = c < kPlanLong ? atomicAdd(&h[c], 1) : 0;
+    const int64_t stride = int64_t(gridDim.x) * kPlanThreads;
+    auto classes = [&](int64_t i, int& c1, int& c2) {
+        const int c = i < nb ? int(cls[i]) : 255;
+        const bool split = c >= kPlanSplitCls && c != 255;
+        c1 = c < kPlanLong ? c : (split ? t.split : -1);   // the segment, or a split bucket's head
+        c2 = split ? c - kPlanSplitCls : -1;               // a split bucket's tail
+    };
+    for (int64_t i0 = int64_t(blockIdx.x) * kPlanThreads; i0 < nb; i0 += stride) {
+        int c1, c2;
+        classes(i0 + threadIdx.x, c1, c2);
+        plan_count(h, c1);
+        plan_count(h, c2 >= 0 ? kPlanSplit : -1);
+        plan_count(h, c2);
+    }
     __syncthreads();
-    if (threadIdx.x < kPlanClasses && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x < kPlanClasses && threadIdx.x != kPlanLong) {
+        base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]) : 0;
+        h[threadIdx.x] = 0;
+    }
     __syncthreads();
-    if (c < kPlanLong) list[L.cstart[c] + base[c] + local] = int(i);
+    for (int64_t i0 = int64_t(blockIdx.x) * kPlanThreads; i0 < nb; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        int c1, c2;
+        classes(i, c1, c2);
+        const int l1 = plan_count(h, c1);
+        const int ls = plan_count(h, c2 >= 0 ? kPlanSplit : -1);
+        const int l2 = plan_count(h, c2);
+        if (c1 >= 0) list[L.cstart[c1] + base[c1] + l1] = int(i) | (c2 >= 0 ? kSegHead : 0);
+        if (c2 >= 0) {
+            list[L.cstart[c2] + base[c2] + l2] = int(i) | kSegTail;
+            split_list[base[kPlanSplit] + ls] = int(i);
+        }
+    }
 }
 
 // Work items of the class-grouped list; workgroup 0 also writes the item
@@ -323,7 +417,27 @@ hipError_t exclusive_scan(int64_t* v, int64_t n, int64_t* sums, hipStream_t s) {
 
 namespace {
 
-constexpr int64_t kPartBlockElems = 65536;   // elements per workgroup of count / scatter
+// Elements per workgroup of count / scatter: 512 KiB of elements while that
+// leaves at least 1024 workgroups, else halved down to 64 KiB (fewer
+// workgroups leave CUs idle at the grid's end; fewer and longer ones shrink
+// the count matrix and its scan).  Elements per workgroup, ms per sort:
+// 2^28 int32 32768 3.94, 65536 3.60, 131072 3.50, 262144 3.81; 2^26 int32
+// 65536 1.06, 131072 1.16; 2^27 uint64 32768 4.06, 65536 4.03, 131072 4.08;
+// 2^25 int64+int32 records 16384 2.81, 32768 2.83, 65536 2.85.
+static int64_t part_block_elems(size_t esz, int64_t n) {
+    int64_t e = int64_t(512 * 1024) / int64_t(esz);
+    const int64_t floor_e = int64_t(64 * 1024) / int64_t(esz);
+    while (e > floor_e && n / e < 1024) e /= 2;
+    return e;
+}
+
+// Most first-level workgroups of any sort of up to n elements (a smaller
+// sort can take smaller workgroups, and so more of them: fewer than 2048
+// whenever it does).
+static int64_t part_blocks_bound(size_t esz, int64_t n) {
+    const int64_t e0 = int64_t(512 * 1024) / int64_t(esz), floor_e = int64_t(64 * 1024) / int64_t(esz);
+    return std::max((n + e0 - 1) / e0, std::min<int64_t>(2048, (n + floor_e - 1) / floor_e));
+}
 constexpr int kPartOneLevel = 256;           // parts up to which one level is used
 
 // Parts of a partition sort of n elements (the caller's MinNbParts /
@@ -336,7 +450,6 @@ struct PartShape {
 };
 
 PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
-    const LargeKernels lk = large_kernels_for(dtype);
     const size_t esz = cs_dtype_size(dtype);
     PartShape sh;
     // Buckets average 1536 elements, 3/4 of a 2048-key tile: with 12-64
@@ -395,7 +508,8 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     const PartShape sh = part_shape(dtype, n, min_parts, max_parts);
     const int p1 = sh.p1, p2 = sh.p2;
     const int64_t parts = sh.parts, m = sh.m, target = sh.target;
-    const int64_t blocks1 = (n + kPartBlockElems - 1) / kPartBlockElems;
+    const int64_t block_elems = part_block_elems(esz, n);
+    const int64_t blocks1 = (n + block_elems - 1) / block_elems;
     const int nb1 = 2 * p1 - 1;
     const int64_t cells1 = int64_t(nb1) * blocks1;
     // long-bucket sort (final buckets longer than the tile)
@@ -425,7 +539,7 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     rc = merge_sort(dtype, samples, m, samples, samples_tmp, tmp);
     if (rc != CS_OK) return rc;
     if (e == hipSuccess)
-        e = pk.prep(samples, int(m), int(parts), p1, p2, gsp, l1sp, n, kPartBlockElems, blocks1, work1, segs1, stream);
+        e = pk.prep(samples, int(m), int(parts), p1, p2, gsp, l1sp, n, block_elems, blocks1, work1, segs1, stream);
     if (e != hipSuccess) return int(e);
 
     // level 1: data -> values (straight into out for one level of an
@@ -480,7 +594,7 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
         if (rc == CS_OK) rc = tmp.get(&d_feq, size_t(fnb));
         if (rc != CS_OK) return rc;
         hipLaunchKernelGGL(k_part_level2, dim3(1), dim3(kL2Threads), 0, stream, counts1, blocks1, nb1, n, p1, p2,
-                           target, kPartBlockElems, work2, segs2, hdr2);
+                           target, block_elems, work2, segs2, hdr2);
         e = hipGetLastError();
         // cells past hdr2[1] are scanned too; no used cell depends on them
         if (e == hipSuccess) e = pk.count(values, gsp, work2, int(blocks2), p2 - 1, counts2, hdr2, stream);
@@ -500,24 +614,28 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     PlanTable t{};
     t.maxb = sc.maxb;
     t.small_max = sc.small_max;
+    t.split = sc.small_max < sc.maxb ? sc.small_max : -1;
     for (int c = 0; c < kPlanClasses; ++c) {
         t.spi[c] = c <= sc.maxb ? std::max(1, sc.spi[c]) : 1;
         t.variant[c] = c <= sc.maxb ? sc.variant[c] : 0;
     }
     unsigned char* cls = nullptr;
-    int *d_list = nullptr, *long_list = nullptr;
+    int *d_list = nullptr, *long_list = nullptr, *split_list = nullptr;
     SegItem* d_items = nullptr;
     rc = tmp.get(&cls, size_t(fnb));
-    if (rc == CS_OK) rc = tmp.get(&d_list, size_t(fnb));
-    if (rc == CS_OK) rc = tmp.get(&d_items, size_t(fnb));
+    if (rc == CS_OK) rc = tmp.get(&d_list, 2 * size_t(fnb));     // a split bucket is listed twice
+    if (rc == CS_OK) rc = tmp.get(&d_items, 2 * size_t(fnb));
     if (rc == CS_OK) rc = tmp.get(&long_list, size_t(nlong_max));
+    if (rc == CS_OK) rc = tmp.get(&split_list, size_t(fnb));
     if (rc != CS_OK) return rc;
     const unsigned g = unsigned((fnb + 255) / 256);
     // the final sort runs in place: equality buckets need no sort
-    hipLaunchKernelGGL(k_plan_classify, dim3(g), dim3(256), 0, stream, d_fst, d_feq, 1, fnb, d_fnb, n,
-                       int64_t(lk.chunk), cls, meta, long_list);
-    hipLaunchKernelGGL(k_plan_fill, dim3(g), dim3(256), 0, stream, cls, fnb, d_fnb, meta, t, d_list);
-    hipLaunchKernelGGL(k_plan_items, dim3(g), dim3(256), 0, stream, meta, t, d_items);
+    const unsigned gp = unsigned(std::min<int64_t>(kPlanGrid, (fnb + kPlanThreads - 1) / kPlanThreads));
+    hipLaunchKernelGGL(k_plan_classify, dim3(gp), dim3(kPlanThreads), 0, stream, d_fst, d_feq, 1, fnb, d_fnb, n,
+                       int64_t(lk.chunk), t.split, cls, meta, long_list);
+    hipLaunchKernelGGL(k_plan_fill, dim3(gp), dim3(kPlanThreads), 0, stream, cls, fnb, d_fnb, meta, t, d_list,
+                       split_list);
+    hipLaunchKernelGGL(k_plan_items, dim3(2 * g), dim3(256), 0, stream, meta, t, d_items);   // up to 2 fnb items
     e = hipGetLastError();
     if (e != hipSuccess) return int(e);
     SegShape ss;
@@ -526,6 +644,9 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     if (ss.grid[kSegMulti] <= 0 || ss.grid[kSegSingle] <= 0) return CS_E_NO_DEVICE;
     e = seg_launch_all(dtype, ss, out, out, d_fst, d_list, d_items, 0, d_counters, stream, meta + 4 * kPlanClasses,
                        false);
+    // the two sorted halves of every split bucket (meta[kPlanSplit] of them)
+    if (e == hipSuccess && t.split >= 0)
+        e = pk.split_merge(out, d_fst, split_list, meta + kPlanSplit, tmp.grid_cap, stream);
     // buckets longer than the tile (meta[kPlanLong] of them): chunk sorts and
     // merge passes between out and values
     if (e == hipSuccess)
@@ -546,9 +667,15 @@ size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_part
     size_t total = 0;
     const int64_t nlong_max = n / lk.chunk + 1;
     const int64_t ntile_max = n / lk.tile + nlong_max;
-    total += 2 * r(size_t(sh.m) * esz);                                   // samples, sorted copy
-    total += merge_scratch_bytes(dtype, sh.m);                            // their merge sort
-    total += r(size_t(sh.parts - 1) * size_t(pk.key_size));               // global splitters
+    // Two levels: p1 = ceil(sqrt(P)) grows with n, while p2 = P / p1 and the
+    // part count p1 * p2 do not always; sizing parts, samples and second-level
+    // buckets by p1 * p1 keeps this bound growing with n, so a workspace
+    // reserved for n holds every smaller sort.
+    const int64_t parts = sh.p2 > 1 ? int64_t(sh.p1) * sh.p1 : sh.parts;
+    const int64_t m = sh.m / sh.parts * parts;
+    total += 2 * r(size_t(m) * esz);                                      // samples, sorted copy
+    total += merge_scratch_bytes(dtype, m);                               // their merge sort
+    total += r(size_t(parts - 1) * size_t(pk.key_size));                  // global splitters
     total += r(size_t(std::max(sh.p1 - 1, 1)) * size_t(pk.key_size));     // first-level splitters
     total += r(size_t(ntile_max + 1) * 8) + r(size_t(nlong_max) * 4);     // long buckets: co-ranks, list
     total += r(size_t(kPlanMeta + 2) * 4);                                // plan counters
@@ -557,19 +684,20 @@ size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_part
         b += r(size_t(cells) * 8) + r(size_t((cells + kScanTile - 1) / kScanTile + 1) * 8);
         return b;
     };
-    const int64_t blocks1 = (n + kPartBlockElems - 1) / kPartBlockElems;
+    const int64_t blocks1 = part_blocks_bound(esz, n);
     const int64_t nb1 = 2 * int64_t(sh.p1) - 1;
     total += level(1, blocks1, nb1 * blocks1);
     int64_t fnb = nb1;
     if (sh.p2 > 1) {
-        const int64_t nbs = 2 * int64_t(sh.p2) - 1;                       // buckets per second-level segment
+        const int64_t nbs = 2 * int64_t(sh.p1) - 1;                       // buckets per second-level segment (bound)
         const int64_t blocks2 = blocks1 + nb1;                            // each segment rounds up once
         fnb = nbs * nb1;
         total += level(nb1, blocks2, nbs * blocks2) + r(4 * 8);           // + device header
     }
     total += r(size_t(fnb + 1) * 8) + r(size_t(fnb));                     // final starts, equality flags
-    // device plan of the final sort
-    total += r(size_t(fnb)) + r(size_t(fnb) * 4) + r(size_t(fnb) * sizeof(SegItem));
+    // device plan of the final sort: classes, list and items (a split bucket
+    // twice), split list
+    total += r(size_t(fnb)) + r(2 * size_t(fnb) * 4) + r(2 * size_t(fnb) * sizeof(SegItem)) + r(size_t(fnb) * 4);
     return total;
 }
 

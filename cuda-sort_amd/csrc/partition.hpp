@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "large.hpp"
+#include "segmented.hpp"
 
 namespace cs {
 
@@ -285,7 +286,7 @@ __device__ __forceinline__ int part_wave_add(int* counter, int b, bool valid) {
 // Raw buffer loads of a workgroup's element range: a position at or past
 // `elems` reads as zero instead of leaving the range, so a tile's loads need
 // no clamping, and their addresses are 32-bit offsets from a scalar base
-// (a range is at most kPartBlockElems elements).
+// (a range is at most 512 KiB).
 template <class S>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t part_rsrc(const S* base, int elems) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<S*>(base), short(0), (elems > 0 ? elems : 0) * int(sizeof(S)),
@@ -344,7 +345,7 @@ k_part_count(const typename Elem<DT>::Storage* src, const typename Elem<DT>::Key
     part_build_lut<NT>(sp, ns, lut, map, wsum);
     __syncthreads();
     // 32-bit element indices relative to the workgroup's range (at most
-    // kPartBlockElems long).  The next tile's loads are issued before this
+    // 512 KiB long).  The next tile's loads are issued before this
     // tile is classified; loads past the end read zero (their results are
     // masked below).
     const S* base = src + w.begin;
@@ -705,6 +706,92 @@ k_part_long_merge(typename Elem<DT>::Storage* out, typename Elem<DT>::Storage* o
     }
 }
 
+// Buckets a little longer than the single-wave tile (2^SM < len <= 2^(SM+1),
+// SM = seg_small_max) are sorted by the persistent kernel as two segments,
+// head [0, 2^SM) and tail [2^SM, len), both in single-wave tiles, instead of
+// one padded tile of 2^(SM+1) spread over several waves; this kernel then
+// merges the two sorted halves of each such bucket in place.  The whole
+// bucket is staged in LDS before anything is written back, every thread
+// finds its co-rank there and merges its outputs into registers (left half
+// first on equal keys), and the result leaves through LDS with consecutive
+// threads on consecutive addresses.  A persistent grid walks the list, each
+// workgroup loading its next bucket while it merges the current one.
+template <int DT>
+__global__ void __launch_bounds__(kMergeThreads)
+k_part_split_merge(typename Elem<DT>::Storage* out, const int64_t* st, const int* split_list, const int* nsplit_p) {
+    using E = Elem<DT>;
+    using S = typename E::Storage;
+    constexpr int H = 1 << seg_small_max<DT>();
+    constexpr int W = 2 * H;
+    constexpr int IPT = W / kMergeThreads;
+    static_assert(W % kMergeThreads == 0, "whole outputs per thread");
+    constexpr int PIN = sizeof(S) == 4 ? 3 : 2;   // input pads, as in the merge passes
+    constexpr int PAD = sizeof(S) == 4 ? 5 : (sizeof(S) == 8 ? 4 : 3);
+    constexpr int LDS_IN = W + (W >> PIN);
+    constexpr int LDS_OUT = W + (W >> PAD);
+    __shared__ S lds[LDS_IN > LDS_OUT ? LDS_IN : LDS_OUT];
+    const int nsplit = *nsplit_p;
+    const int G = int(gridDim.x);
+    const int tid = int(threadIdx.x);
+    // the next bucket's elements are loaded into registers while this one
+    // is merged
+    S nxt[IPT];
+    int64_t nb0 = 0;
+    int ntotal = 0;
+    auto fetch = [&](int q) {
+        const int b = split_list[q];
+        nb0 = st[b];
+        ntotal = int(st[b + 1] - nb0);   // H < ntotal <= W
+        #pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int x = tid + k * kMergeThreads;
+            if (x < ntotal) nxt[k] = out[nb0 + x];
+        }
+    };
+    if (int(blockIdx.x) < nsplit) fetch(int(blockIdx.x));
+    for (int q = int(blockIdx.x); q < nsplit; q += G) {
+        const int total = ntotal;
+        S* base = out + nb0;
+        #pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int x = tid + k * kMergeThreads;
+            if (x < total) lds[x + (x >> PIN)] = nxt[k];
+        }
+        __syncthreads();
+        if (q + G < nsplit) fetch(q + G);
+        const int la = H, lb = total - H;
+        const int dt0 = tid * IPT < total ? tid * IPT : total;
+        int i = lds_corank<E, PIN>(lds, la, lb, dt0);
+        int j = dt0 - i;
+        const int ia0 = i < total ? i : 0;
+        const int jb0 = la + j < total ? la + j : 0;
+        S cur_a = lds[ia0 + (ia0 >> PIN)];
+        S cur_b = lds[jb0 + (jb0 >> PIN)];
+        S outv[IPT];
+        #pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const bool take_a = j >= lb || (i < la && !(E::key(cur_b) < E::key(cur_a)));
+            outv[k] = take_a ? cur_a : cur_b;
+            i += take_a ? 1 : 0;
+            j += take_a ? 0 : 1;
+            int r = take_a ? i : la + j;   // refill of the side taken (unused once it is exhausted)
+            r = r < total ? r : 0;
+            const S nv = lds[r + (r >> PIN)];
+            cur_a = take_a ? nv : cur_a;
+            cur_b = take_a ? cur_b : nv;
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int x = dt0 + k;
+            if (x < total) lds[x + (x >> PAD)] = outv[k];
+        }
+        __syncthreads();
+        for (int x = tid; x < total; x += kMergeThreads) base[x] = lds[x + (x >> PAD)];
+        __syncthreads();   // LDS reused by the next bucket
+    }
+}
+
 struct PartKernels {
     hipError_t (*sample)(const void* src, int64_t n, void* samples, int m, hipStream_t s) = nullptr;
     // k_part_prep
@@ -721,6 +808,9 @@ struct PartKernels {
     // device plan (grids of at most grid_cap workgroups).
     hipError_t (*long_sort)(const void* src, void* out, void* other, const int64_t* st, const int* long_list,
                             const int* nlong, int64_t* cor, int grid_cap, hipStream_t s) = nullptr;
+    // k_part_split_merge over the split buckets of a device plan.
+    hipError_t (*split_merge)(void* out, const int64_t* st, const int* split_list, const int* nsplit, int grid_cap,
+                              hipStream_t s) = nullptr;
     // Co-rank search + merge pass (or a plain copy with copy_only); writes
     // the co-ranks into items.
     hipError_t (*seg_merge)(const void* src, void* dst, MergeTile* items, int nitems, int64_t run,
@@ -767,6 +857,19 @@ PartKernels part_kernels() {
         hipLaunchKernelGGL(k_part_scatter<DT>, dim3(unsigned(blocks)), dim3(part_scatter_threads<DT>()), lds, s,
                            static_cast<const S*>(src), static_cast<const K*>(sp), work, max_ns, offsets,
                            static_cast<S*>(dst), nwork);
+        return hipGetLastError();
+    };
+    k.split_merge = [](void* out, const int64_t* st, const int* split_list, const int* nsplit, int grid_cap,
+                       hipStream_t s) {
+        // eight workgroups per CU walk the list (any grid is correct)
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        int grid = std::max(1, 8 * cus);
+        if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
+        hipLaunchKernelGGL(k_part_split_merge<DT>, dim3(unsigned(grid)), dim3(kMergeThreads), 0, s,
+                           static_cast<S*>(out), st, split_list, nsplit);
         return hipGetLastError();
     };
     k.long_sort = [](const void* src, void* out, void* other, const int64_t* st, const int* long_list,

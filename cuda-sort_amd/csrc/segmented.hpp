@@ -133,6 +133,29 @@ __device__ __forceinline__ void lds_segment_sort(const typename E::Storage* src,
     __syncthreads();
 }
 
+// A list entry names a segment s (off[s] .. off[s+1]) or one half of it: a
+// planner may cut a segment a little longer than 2^small_max into its head
+// (the first 2^small_max elements, kSegHead) and its tail (the rest,
+// kSegTail), so that both halves take the single-wave tiles; the two sorted
+// halves are merged afterwards.
+constexpr int kSegHead = 1 << 30;
+constexpr int kSegTail = 1 << 29;
+constexpr int kSegIndex = kSegTail - 1;
+
+template <int DT>
+__device__ __forceinline__ void seg_range(const int64_t* off, int entry, int64_t& b0, int& len) {
+    constexpr int64_t H = int64_t(1) << seg_small_max<DT>();
+    const int s = entry & kSegIndex;
+    b0 = off[s];
+    int64_t l = off[s + 1] - b0;
+    if (entry & kSegHead) l = H;
+    if (entry & kSegTail) {
+        b0 += H;
+        l -= H;
+    }
+    len = int(l);
+}
+
 // The body of every (bucket, variant) is a separate, non-inlined function so
 // that each gets its own register allocation: inlined into one kernel, the
 // allocator sized the whole persistent kernel for its worst bucket (more
@@ -162,9 +185,9 @@ __device__ __attribute__((noinline)) void seg_tile_item(global_ptr<const typenam
     const int group = tid >> C::LOGT;
     const int t = tid & (C::T - 1);
     const bool active = group < count;
-    const int s = active ? list[first + group] : 0;
-    const int64_t b0 = active ? off[s] : 0;
-    const int len = active ? int(off[s + 1] - b0) : 0;
+    int64_t b0 = 0;
+    int len = 0;
+    if (active) seg_range<DT>(off, list[first + group], b0, len);
     tile_sort<E, B, LE>(in + b0, out + b0, len, t, lk + (C::NTRANS ? group * C::REGION : 0),
                         lp + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
 }
@@ -176,10 +199,10 @@ __device__ __attribute__((noinline)) void seg_lds_item(global_ptr<const typename
                                                        global_ptr<const int64_t> goff, global_ptr<const int> glist,
                                                        int first, typename Elem<DT>::Key* lk, int32_t* lp) {
     using S = typename Elem<DT>::Storage;
-    const int s = ((const int*)glist)[first];
-    const int64_t* off = (const int64_t*)goff;
-    const int64_t b0 = off[s];
-    lds_segment_sort<Elem<DT>>((const S*)gin + b0, (S*)gout + b0, int(off[s + 1] - b0), B, lk, lp);
+    int64_t b0 = 0;
+    int len = 0;
+    seg_range<DT>((const int64_t*)goff, ((const int*)glist)[first], b0, len);
+    lds_segment_sort<Elem<DT>>((const S*)gin + b0, (S*)gout + b0, len, B, lk, lp);
 }
 
 template <int DT, int B>

@@ -269,6 +269,7 @@ int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* fo
                     std::vector<SegItem>& items, int64_t* max_len, int* variants, int64_t* bucket_segments) {
     const SegKernels k = seg_kernels_for(dtype);
     const int maxb = k.max_logw;
+    if (segs > kSegIndex) return CS_E_CAPACITY;   // list entries keep two flag bits
     // counting sort of the segments by bucket, largest bucket first
     thread_local std::vector<unsigned char> bkt;   // kept mapped between calls
     bkt.resize(static_cast<size_t>(segs));

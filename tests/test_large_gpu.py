@@ -22,7 +22,7 @@ def run_large(cs, orc, rt, dtype, dist, n, method, seed=42, **kw):
     elif method == "partition":
         cs.sort_partition(dtype, buf.ptr, n, **kw)
     elif method == "partition_ws":
-        ws = cs.Workspace(dtype, n)
+        ws = kw.pop("ws", None) or cs.Workspace(dtype, n)
         cs.sort_partition_ws(dtype, buf.ptr, n, ws, **kw)
     elif method == "hybrid":
         cs.sort_hybrid(dtype, buf.ptr, n, kw.get("ws"))
@@ -91,14 +91,44 @@ def test_config4_full(cs, orc, gpu, dtype, n, dist):
 @pytest.mark.parametrize("dtype", ["I32", "F32", "U64", "F64", "REC_I64_I32"])
 def test_partition_level_boundaries(cs, orc, gpu, dtype):
     """Sizes around the one-level / two-level switch of the partition path
-    (256 parts of 3/8 of the on-chip tile) and small part bounds that force
-    buckets longer than the tile onto the long-bucket chunk sort + merge passes."""
+    (256 parts of 1536 elements) and small part bounds that force buckets
+    longer than the chunk onto the long-bucket chunk sort + merge passes."""
     dt = getattr(cs, dtype)
-    target = (8192 if cs.DTYPE_SIZE[dt] == 4 else 4096) * 3 // 8
+    target = 1536
     for n in (256 * target - 1, 256 * target, 256 * target + 1, 300 * target + 17):
         run_large(cs, orc, gpu, dt, orc.D_UNIFORM, n, "partition", seed=n)
     run_large(cs, orc, gpu, dt, orc.D_DUP50, 300 * target + 17, "partition", max_parts=4)
     run_large(cs, orc, gpu, dt, orc.D_UNIFORM, 70000, "partition", min_parts=4000)
+
+
+@pytest.mark.parametrize("dtype", ["I32", "U64", "F32", "REC_I32_I32", "REC_I64_I32"])
+def test_partition_split_buckets(cs, orc, gpu, dtype):
+    """Final buckets a little longer than the single-wave tile (2048
+    elements, 1024 for 16-byte records) are sorted as a head and a tail and
+    merged: part bounds that put most buckets there, with tails from one
+    element to a full head, heavy duplicates (equality buckets stay out of
+    it) and presorted runs, one level and two, in place and through a
+    workspace."""
+    dt = getattr(cs, dtype)
+    head = 1024 if cs.DTYPE_SIZE[dt] == 16 else 2048
+    for n, parts in (((1 << 21) + 3, ((1 << 21) + 3) // (head * 3 // 2)),   # two levels, buckets ~1.5 heads
+                     (200 * head + 1, 200),                                   # one level, buckets ~1 head
+                     (150 * head + 77, 100)):                                # one level, buckets ~1.5 heads
+        for d in ("D_UNIFORM", "D_DUP50", "D_PRESORTED_RUNS", "D_DISTINCT16"):
+            run_large(cs, orc, gpu, dt, getattr(orc, d), n, "partition", seed=n + len(d), max_parts=parts)
+        run_large(cs, orc, gpu, dt, orc.D_UNIFORM, n, "partition_ws", seed=n + 5, max_parts=parts)
+
+
+def test_workspace_holds_every_smaller_sort(cs, orc, gpu):
+    """A workspace reserved for n elements serves every smaller partition
+    sort: below 2^26 8-byte elements the count / scatter workgroups shrink
+    (so there can be more of them), the two-level part counts do not grow
+    evenly with n, and 256 parts or fewer take one level with more samples
+    per part."""
+    cap = 1 << 26
+    ws = cs.Workspace(cs.U64, cap)
+    for n in (cap, cap - 3, (1 << 25) + 1, 256 * 1536, 256 * 1536 + 1, 100003):
+        run_large(cs, orc, gpu, cs.U64, orc.D_UNIFORM, n, "partition_ws", seed=n, ws=ws)
 
 
 @pytest.mark.parametrize("dtype", ["I32", "U64", "REC_I32_I32", "REC_I64_I32"])
