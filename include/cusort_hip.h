@@ -226,7 +226,10 @@ int cs_segmented_set_variant(int dtype, int32_t bucket, int32_t variant);
  * n_streams worker streams with completion events.  The _ws entry points
  * and cs_sort_hybrid with a workspace of capacity >= n allocate no device
  * memory (cs_memory_stats counts the library's allocations); they fail with
- * CS_E_WORKSPACE when the workspace is too small or of another dtype.  Calls
+ * CS_E_WORKSPACE when the workspace is too small or of another dtype.  A
+ * workspace of capacity c holds every sort of n <= c elements with the
+ * default part count or any max_parts; a min_parts above the default can
+ * need more (CS_E_WORKSPACE).  Calls
  * that share a workspace are ordered through an event (each waits for the
  * previous one on its own stream); reserve and destroy free and allocate in
  * stream order behind them, without synchronising the host or the device.
