@@ -447,6 +447,12 @@ struct PartShape {
     int64_t target = 0;
     int p1 = 0, p2 = 1;
     int64_t parts = 0, m = 0;
+    // two-stage sampling (two levels, when part j's sample of s2 * p2 keys
+    // fits one chunk): m1 = s1 * p1 keys for the first level only, then s2
+    // keys per second-level part from each first-level bucket
+    bool two_stage = false;
+    int s1 = 0, s2 = 0;
+    int64_t m1 = 0;
 };
 
 PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
@@ -480,6 +486,20 @@ PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
     // (8-byte keys make the bucket sort dearer per spilled key), so wider
     // elements keep 32.
     sh.m = sh.parts * (sh.p2 > 1 ? (esz == 4 ? 12 : 32) : 64);
+    // Two-stage sampling where part j's sample fits one chunk: 16 keys per
+    // first-level part, 12 per second-level part from each first-level
+    // bucket.  2^28 int32 (ms per sort): one global sample 3.50; two-stage
+    // with 16 / 32 / 64 first-level and 12 second-level samples per part
+    // 3.42 / 3.43 / 3.48, with 8 / 16 second-level 3.45 / 3.43; 2^26 int32
+    // 1.06 -> 0.99, 2^26 int32+int32 records 1.96 -> 1.88, 2^27 uint64
+    // 4.03 -> 3.92; 2^28 int32 in presorted runs 3.70 either way.  The
+    // global sample's merge sort (eight passes of small, latency-bound
+    // kernels) gives way to one chunk sort of all parts' samples.
+    const LargeKernels lk = large_kernels_for(dtype);
+    sh.s1 = 16;
+    sh.s2 = 12;
+    sh.two_stage = sh.p2 > 1 && int64_t(sh.s2) * sh.p2 <= lk.chunk;
+    sh.m1 = int64_t(sh.s1) * sh.p1;
     return sh;
 }
 
@@ -487,15 +507,18 @@ PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
 
 // Sorts data[0, n) into out (out may equal data) with `values` as scratch of
 // at least n elements and the call's other temporaries from `tmp`.  Sample
-// sort into buckets of ~3/8 of the on-chip tile: one partition level
-// (data -> values) for up to kPartOneLevel parts, else two (data -> values ->
-// out) with p1 x p2 parts taken from one global splitter list, so the second
-// level needs no sampling of its own.  The buckets are then sorted as one
-// ragged batch (values -> out, or in place in out), planned on the device;
-// the rare bucket longer than a tile is sorted by k_part_long's chunk sort
-// and merge passes.  Everything is laid out on the device (the second
-// level's work list, the final plan, the long buckets' chunks and tiles):
-// the call enqueues its work on the stream and returns without waiting.
+// sort into buckets of 1536 elements: one partition level (data -> values)
+// for up to kPartOneLevel parts, else two (data -> values -> out) with p1 x
+// p2 parts in one splitter list: the first-level splitters from a small
+// sample, each first-level bucket's second-level splitters from its own
+// sample (or, where that sample would not fit a chunk, all from one global
+// sample).  The buckets are then sorted as one ragged batch in place in
+// out, planned on the device; buckets just over the single-wave tile are
+// sorted as head and tail and merged, and the rare bucket longer than a
+// chunk by chunk sorts and merge passes.  Everything is laid out on the
+// device (the second level's splitters and work list, the final plan, the
+// long buckets' chunks and tiles): the call enqueues its work on the stream
+// and returns without waiting.
 int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, size_t values_elems, int min_parts,
                    int max_parts, Scratch& tmp) {
     const LargeKernels lk = large_kernels_for(dtype);
@@ -516,13 +539,15 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     const int64_t nlong_max = n / lk.chunk + 1;
     const int64_t ntile_max = n / lk.tile + nlong_max;
 
-    unsigned char *samples = nullptr, *samples_tmp = nullptr, *gsp = nullptr, *l1sp = nullptr;
+    unsigned char *samples = nullptr, *samples_tmp = nullptr, *gsp = nullptr, *l1sp = nullptr, *smp2 = nullptr;
     int64_t* cor = nullptr;
     PartBlock* work1 = nullptr;
     PartSeg* segs1 = nullptr;
     int64_t *counts1 = nullptr, *sums1 = nullptr;
-    int rc = tmp.get(&samples, size_t(m) * esz);
-    if (rc == CS_OK) rc = tmp.get(&samples_tmp, size_t(m) * esz);
+    const int64_t msort = sh.two_stage ? sh.m1 : m;   // keys sampled and sorted before the first level
+    int rc = tmp.get(&samples, size_t(msort) * esz);
+    if (rc == CS_OK) rc = tmp.get(&samples_tmp, size_t(msort) * esz);
+    if (rc == CS_OK && sh.two_stage) rc = tmp.get(&smp2, size_t(p1) * size_t(lk.chunk) * esz);
     if (rc == CS_OK) rc = tmp.get(&gsp, size_t(parts - 1) * size_t(pk.key_size));
     if (rc == CS_OK && p2 > 1) rc = tmp.get(&l1sp, size_t(std::max(p1 - 1, 1)) * size_t(pk.key_size));
     if (rc == CS_OK) rc = tmp.get(&cor, size_t(ntile_max) + 1);
@@ -534,11 +559,13 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
 
     // hashed sample sorted by the merge path; splitters and the first
     // level's work list from it
-    hipError_t e = pk.sample(data, n, samples, int(m), stream);
+    hipError_t e = pk.sample(data, n, samples, int(msort), stream);
     if (e != hipSuccess) return int(e);
-    rc = merge_sort(dtype, samples, m, samples, samples_tmp, tmp);
+    rc = merge_sort(dtype, samples, msort, samples, samples_tmp, tmp);
     if (rc != CS_OK) return rc;
-    if (e == hipSuccess)
+    if (sh.two_stage)   // first-level splitters only (one level of p1 parts from the small sample)
+        e = pk.prep(samples, int(msort), p1, p1, 1, l1sp, nullptr, n, block_elems, blocks1, work1, segs1, stream);
+    else
         e = pk.prep(samples, int(m), int(parts), p1, p2, gsp, l1sp, n, block_elems, blocks1, work1, segs1, stream);
     if (e != hipSuccess) return int(e);
 
@@ -555,6 +582,14 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     if (p2 == 1 && dst1 != out) {
         rc = cs_device_copy(out, values, size_t(n) * esz, stream);
         if (rc != CS_OK) return rc;
+    }
+    // two-stage sampling: the second-level splitters from each first-level
+    // bucket's own sample (counts1 now holds the bucket starts)
+    if (sh.two_stage) {
+        e = pk.l2_gather(values, counts1, blocks1, p1, p2, sh.s2, lk.chunk, n, smp2, stream);
+        if (e == hipSuccess) e = lk.chunk_sort(smp2, smp2, int64_t(p1) * lk.chunk, stream);
+        if (e == hipSuccess) e = pk.l2_pick(smp2, counts1, blocks1, p1, p2, sh.s2, lk.chunk, n, l1sp, gsp, stream);
+        if (e != hipSuccess) return int(e);
     }
 
     // final plan's counters (meta, then the persistent kernels' two), cleared
@@ -673,8 +708,13 @@ size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_part
     // reserved for n holds every smaller sort.
     const int64_t parts = sh.p2 > 1 ? int64_t(sh.p1) * sh.p1 : sh.parts;
     const int64_t m = sh.m / sh.parts * parts;
-    total += 2 * r(size_t(m) * esz);                                      // samples, sorted copy
-    total += merge_scratch_bytes(dtype, m);                               // their merge sort
+    // samples, their sorted copy and merge sort: one global sample, or the
+    // small first-level sample and a chunk per first-level part (the larger)
+    const size_t global_smp = 2 * r(size_t(m) * esz) + merge_scratch_bytes(dtype, m);
+    const size_t staged_smp = sh.p2 > 1 ? 2 * r(size_t(sh.m1) * esz) + merge_scratch_bytes(dtype, sh.m1) +
+                                              r(size_t(sh.p1) * size_t(lk.chunk) * esz)
+                                        : 0;
+    total += std::max(global_smp, staged_smp);
     total += r(size_t(parts - 1) * size_t(pk.key_size));                  // global splitters
     total += r(size_t(std::max(sh.p1 - 1, 1)) * size_t(pk.key_size));     // first-level splitters
     total += r(size_t(ntile_max + 1) * 8) + r(size_t(nlong_max) * 4);     // long buckets: co-ranks, list

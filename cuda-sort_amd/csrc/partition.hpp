@@ -114,6 +114,70 @@ __global__ void k_part_prep(const typename Elem<DT>::Storage* sorted, int m, int
     }
 }
 
+// Second-level splitters drawn from each first-level bucket (two-stage
+// sampling): first-level part j (bucket 2j, keys strictly between
+// first-level splitters j-1 and j) gets s2 * p2 keys sampled at hashed
+// positions of its own range in the first level's output, sorted as one
+// chunk (the rest of the chunk padded with the maximum key), and every
+// s2-th of them becomes one of its p2 - 1 splitters.  The splitter list
+// keeps the layout of one global list: part j's second-level splitters at
+// j * p2 .. j * p2 + p2 - 2 and first-level splitter j at (j + 1) * p2 - 1,
+// so the second level is laid out and run exactly as before.  A bucket's
+// own sample follows its actual key range, so its final buckets come out at
+// the target size whatever size the first level gave it.
+// starts: the first level's scanned count matrix (bucket b starts at
+// starts[b * blocks1]).
+template <int DT>
+__device__ __forceinline__ void part_l2_range(const int64_t* starts, int64_t blocks1, int p1, int64_t n, int j,
+                                              int64_t& b0, int64_t& len) {
+    const int b = 2 * j;
+    b0 = starts[int64_t(b) * blocks1];
+    const int64_t e = b + 1 < 2 * p1 - 1 ? starts[int64_t(b + 1) * blocks1] : n;
+    len = e - b0;
+}
+
+// smp[j * chunk + i]: i < s2 * p2 sampled keys of part j (none if the part
+// is empty), the rest of the chunk the maximum key.  One thread per slot.
+template <int DT>
+__global__ void k_part_l2_gather(const typename Elem<DT>::Storage* src, const int64_t* starts, int64_t blocks1,
+                                 int p1, int p2, int s2, int chunk, int64_t n, typename Elem<DT>::Storage* smp) {
+    using E = Elem<DT>;
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= int64_t(p1) * chunk) return;
+    const int j = int(q / chunk), i = int(q - int64_t(j) * chunk);
+    int64_t b0 = 0, len = 0;
+    part_l2_range<DT>(starts, blocks1, p1, n, j, b0, len);
+    if (i < s2 * p2 && len > 0) {
+        const uint32_t x = uint32_t(i) * 2654435761u ^ uint32_t(j) * 0x9e3779b9u;
+        const uint64_t h = (uint64_t(part_hash(x)) << 32) | part_hash(x ^ 0x85ebca6bu);
+        smp[q] = src[b0 + int64_t(h % uint64_t(len))];
+    } else {
+        smp[q] = E::make(E::kPad, INT32_MAX);
+    }
+}
+
+// gsp[j * p2 + k] (k < p2 - 1) from part j's sorted sample, and
+// gsp[(j + 1) * p2 - 1] = first-level splitter j.  An empty part repeats
+// its lower bound (its segment is carried through unsplit and never reads
+// them).  One thread per splitter.
+template <int DT>
+__global__ void k_part_l2_pick(const typename Elem<DT>::Storage* smp, const int64_t* starts, int64_t blocks1, int p1,
+                               int p2, int s2, int chunk, int64_t n, const typename Elem<DT>::Key* l1sp,
+                               typename Elem<DT>::Key* gsp) {
+    using E = Elem<DT>;
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= int64_t(p1) * p2 - 1) return;
+    const int j = int(q / p2), k = int(q - int64_t(j) * p2);
+    if (k == p2 - 1) {
+        gsp[q] = l1sp[j];
+        return;
+    }
+    int64_t b0 = 0, len = 0;
+    part_l2_range<DT>(starts, blocks1, p1, n, j, b0, len);
+    if (len > 0) gsp[q] = E::key(smp[int64_t(j) * chunk + int64_t(k + 1) * s2 - 1]);
+    else gsp[q] = j > 0 ? l1sp[j - 1] : l1sp[0];
+}
+
 // Classification.  A lookup table indexed by kPartLutBits bits of the key's
 // order-preserving unsigned image, taken relative to the segment's lower
 // bound and scaled to its width, narrows the splitter search to the
@@ -798,6 +862,11 @@ struct PartKernels {
     hipError_t (*prep)(const void* sorted_samples, int m, int parts, int p1, int p2, void* gsp, void* l1sp, int64_t n,
                        int64_t block_elems, int64_t blocks1, PartBlock* work1, PartSeg* segs1,
                        hipStream_t s) = nullptr;
+    // k_part_l2_gather + k_part_l2_pick around the caller's chunk sort
+    hipError_t (*l2_gather)(const void* src, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2,
+                            int chunk, int64_t n, void* smp, hipStream_t s) = nullptr;
+    hipError_t (*l2_pick)(const void* smp, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2, int chunk,
+                          int64_t n, const void* l1sp, void* gsp, hipStream_t s) = nullptr;
     // count / scatter over `blocks` work items; with nwork (device) only the
     // first *nwork of them exist and the rest of the grid returns at once.
     hipError_t (*count)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
@@ -838,6 +907,21 @@ PartKernels part_kernels() {
         hipLaunchKernelGGL(k_part_prep<DT>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
                            static_cast<const S*>(sorted), m, parts, p1, p2, static_cast<K*>(gsp),
                            static_cast<K*>(l1sp), n, block_elems, blocks1, work1, segs1);
+        return hipGetLastError();
+    };
+    k.l2_gather = [](const void* src, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2, int chunk,
+                     int64_t n, void* smp, hipStream_t s) {
+        const int64_t threads = int64_t(p1) * chunk;
+        hipLaunchKernelGGL(k_part_l2_gather<DT>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const S*>(src), starts, blocks1, p1, p2, s2, chunk, n, static_cast<S*>(smp));
+        return hipGetLastError();
+    };
+    k.l2_pick = [](const void* smp, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2, int chunk,
+                   int64_t n, const void* l1sp, void* gsp, hipStream_t s) {
+        const int64_t threads = int64_t(p1) * p2;
+        hipLaunchKernelGGL(k_part_l2_pick<DT>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const S*>(smp), starts, blocks1, p1, p2, s2, chunk, n,
+                           static_cast<const K*>(l1sp), static_cast<K*>(gsp));
         return hipGetLastError();
     };
     k.count = [](const void* src, const void* sp, const PartBlock* work, int blocks, int max_ns, int64_t* counts,
