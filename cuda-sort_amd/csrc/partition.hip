@@ -13,9 +13,17 @@
 
 namespace cs {
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_sums(const int64_t* v, int64_t n, int64_t* sums) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(const int64_t* v, int64_t n, int64_t* sums,
+                                                            const int64_t* d_n) {
     __shared__ int64_t part[kScanThreads];
     const int64_t b0 = int64_t(blockIdx.x) * kScanTile;
+    if (d_n) {   // count known on the device; blocks past it sum to zero
+        n = *d_n < n ? *d_n : n;
+        if (b0 >= n) {
+            if (threadIdx.x == 0) sums[blockIdx.x] = 0;
+            return;
+        }
+    }
     int64_t s = 0;
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t i = b0 + int64_t(k) * kScanThreads + threadIdx.x;
@@ -55,9 +63,14 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_top(int64_t* sums, int64_
 }
 
 // Exclusive scan of each tile, offset by the tile's scanned sum.
-__global__ void __launch_bounds__(kScanThreads) k_scan_apply(int64_t* v, int64_t n, const int64_t* sums) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(int64_t* v, int64_t n, const int64_t* sums,
+                                                             const int64_t* d_n) {
     __shared__ int64_t buf[kScanThreads];
     const int64_t b0 = int64_t(blockIdx.x) * kScanTile;
+    if (d_n) {
+        n = *d_n < n ? *d_n : n;
+        if (b0 >= n) return;
+    }
     int64_t vals[kScanItems];
     int64_t s = 0;
     for (int k = 0; k < kScanItems; ++k) {           // thread owns kScanItems consecutive values
@@ -94,18 +107,23 @@ __global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64
 
 // Second partition level, laid out on the device from the first level's
 // scanned count matrix (bucket b starts at cell (b, 0) = counts1[b *
-// blocks1]; no host round trip): first-level bucket b becomes segment b.  An equality bucket (odd b) or a bucket no longer than `target` is
-// carried through as one bucket; any other is split by the p2 - 1 global
-// splitters inside it (first-level splitter j is global splitter
-// (j + 1) p2 - 1).  Empty buckets get no workgroup and no bucket, so the
-// final buckets stay in array order.  One workgroup; writes the segment
-// table (nb1 entries), the count / scatter work list and
-// hdr = {work items, count-matrix cells, final buckets}.
+// blocks1]; no host round trip): first-level bucket b becomes segment b.
+// An equality bucket (odd b) is carried through as one bucket.  With one
+// global splitter list (p2cap = 0) a bucket no longer than `target` is
+// carried through too and any other is split by the p2 - 1 splitters
+// inside it (first-level splitter j is list entry (j + 1) p2 - 1); with
+// two-stage sampling (p2cap > 0) part j is split by its own part_p2(len) - 1
+// splitters at j * p2cap (first-level splitter j at (j + 1) p2cap - 1).
+// Empty buckets get no workgroup and no bucket, so the final buckets stay in
+// array order.  One workgroup; writes the segment table (nb1 entries), the
+// count / scatter work list and hdr = {work items, count-matrix cells, final
+// buckets}.
 constexpr int kL2Threads = 1024;
 
 __global__ void __launch_bounds__(kL2Threads)
-k_part_level2(const int64_t* counts1, int64_t blocks1, int nb1, int64_t n, int p1, int p2, int64_t target,
+k_part_level2(const int64_t* counts1, int64_t blocks1, int nb1, int64_t n, int p1, int p2, int p2cap, int64_t target,
               int64_t block_elems, PartBlock* work, PartSeg* segs, int64_t* hdr) {
+    const int stride = p2cap > 0 ? p2cap : p2;   // splitter-list entries per first-level part
     __shared__ int64_t sc[3][kL2Threads];
     const int tid = int(threadIdx.x);
     const int per = (nb1 + kL2Threads - 1) / kL2Threads;
@@ -120,7 +138,8 @@ k_part_level2(const int64_t* counts1, int64_t blocks1, int nb1, int64_t n, int p
         g.begin = counts1[int64_t(b) * blocks1];
         g.end = b + 1 < nb1 ? counts1[int64_t(b + 1) * blocks1] : n;
         const int64_t len = g.end - g.begin;
-        g.ns = ((b & 1) || len <= target) ? 0 : p2 - 1;
+        if (p2cap > 0) g.ns = (b & 1) ? 0 : part_p2(len, target, p2cap) - 1;   // own sample: own part count
+        else g.ns = ((b & 1) || len <= target) ? 0 : p2 - 1;
         g.blocks = len > 0 ? (len + block_elems - 1) / block_elems : 0;
         g.nbs = len > 0 ? 2 * g.ns + 1 : 0;
         return g;
@@ -155,9 +174,9 @@ k_part_level2(const int64_t* counts1, int64_t blocks1, int nb1, int64_t n, int p
         for (int64_t k = 0; k < g.blocks; ++k) {
             const int64_t x0 = g.begin + k * block_elems;
             work[blk + k] = PartBlock{x0, x0 + block_elems < g.end ? x0 + block_elems : g.end, cel + k,
-                                      int32_t(g.blocks), split ? j * p2 : 0, g.ns,
-                                      split && j > 0 ? j * p2 - 1 : -1, split && j < p1 - 1 ? (j + 1) * p2 - 1 : -1,
-                                      0};
+                                      int32_t(g.blocks), split ? j * stride : 0, g.ns,
+                                      split && j > 0 ? j * stride - 1 : -1,
+                                      split && j < p1 - 1 ? (j + 1) * stride - 1 : -1, 0};
         }
         blk += g.blocks;
         bkt += g.nbs;
@@ -405,11 +424,13 @@ PartKernels part_kernels_for(int dtype) {
 
 namespace {
 
-hipError_t exclusive_scan(int64_t* v, int64_t n, int64_t* sums, hipStream_t s) {
+// n bounds the count; with d_n the count is *d_n (known on the device), and
+// the workgroups past it return at once.
+hipError_t exclusive_scan(int64_t* v, int64_t n, int64_t* sums, hipStream_t s, const int64_t* d_n = nullptr) {
     const int64_t blocks = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_sums, dim3(unsigned(blocks)), dim3(kScanThreads), 0, s, v, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(unsigned(blocks)), dim3(kScanThreads), 0, s, v, n, sums, d_n);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, sums, blocks);
-    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(blocks)), dim3(kScanThreads), 0, s, v, n, sums);
+    hipLaunchKernelGGL(k_scan_apply, dim3(unsigned(blocks)), dim3(kScanThreads), 0, s, v, n, sums, d_n);
     return hipGetLastError();
 }
 
@@ -447,12 +468,14 @@ struct PartShape {
     int64_t target = 0;
     int p1 = 0, p2 = 1;
     int64_t parts = 0, m = 0;
-    // two-stage sampling (two levels, when part j's sample of s2 * p2 keys
-    // fits one chunk): m1 = s1 * p1 keys for the first level only, then s2
-    // keys per second-level part from each first-level bucket
+    // two-stage sampling (two levels): m1 = s1 * p1 keys for the first
+    // level only, then s2 keys per second-level part from each first-level
+    // bucket, which gets part_p2(len, target2, p2cap) parts (p2cap * s2 keys
+    // fit one chunk; target2 folds a caller's part bound into the bucket
+    // size)
     bool two_stage = false;
-    int s1 = 0, s2 = 0;
-    int64_t m1 = 0;
+    int s1 = 0, s2 = 0, p2cap = 0;
+    int64_t m1 = 0, target2 = 0;
 };
 
 PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
@@ -486,21 +509,47 @@ PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
     // (8-byte keys make the bucket sort dearer per spilled key), so wider
     // elements keep 32.
     sh.m = sh.parts * (sh.p2 > 1 ? (esz == 4 ? 12 : 32) : 64);
-    // Two-stage sampling where part j's sample fits one chunk: 16 keys per
-    // first-level part, 12 per second-level part from each first-level
-    // bucket.  2^28 int32 (ms per sort): one global sample 3.50; two-stage
-    // with 16 / 32 / 64 first-level and 12 second-level samples per part
-    // 3.42 / 3.43 / 3.48, with 8 / 16 second-level 3.45 / 3.43; 2^26 int32
-    // 1.06 -> 0.99, 2^26 int32+int32 records 1.96 -> 1.88, 2^27 uint64
-    // 4.03 -> 3.92; 2^28 int32 in presorted runs 3.70 either way.  The
-    // global sample's merge sort (eight passes of small, latency-bound
-    // kernels) gives way to one chunk sort of all parts' samples.
+    // Two-stage sampling for two levels: 16 keys per first-level part, 12
+    // per second-level part from each first-level bucket.  2^28 int32 (ms
+    // per sort): one global sample 3.50; two-stage with p2 fixed and 16 / 32
+    // / 64 first-level and 12 second-level samples per part 3.42 / 3.43 /
+    // 3.48, with 8 / 16 second-level 3.45 / 3.43; 2^26 int32 1.06 -> 0.99,
+    // 2^26 int32+int32 records 1.96 -> 1.88, 2^27 uint64 4.03 -> 3.92; 2^28
+    // int32 in presorted runs 3.70 either way.  The global sample's merge
+    // sort (eight passes of small, latency-bound kernels) gives way to one
+    // chunk sort of all parts' samples.  Each first-level bucket then takes
+    // as many second-level parts as its own size needs (the small
+    // first-level sample leaves those sizes ~25 % apart).
     const LargeKernels lk = large_kernels_for(dtype);
     sh.s1 = 16;
     sh.s2 = 12;
-    sh.two_stage = sh.p2 > 1 && int64_t(sh.s2) * sh.p2 <= lk.chunk;
+    sh.p2cap = std::min(maxp, lk.chunk / sh.s2);
+    sh.two_stage = sh.p2 > 1 && sh.p2cap > 1;
     sh.m1 = int64_t(sh.s1) * sh.p1;
+    sh.target2 = std::max<int64_t>(1, (n + sh.parts - 1) / sh.parts);
     return sh;
+}
+
+// Second-level sizes: splitter-list entries per first-level part, buckets
+// per segment at most, final buckets at most and count-matrix cells at most
+// (the grids and temporaries are sized by them; the device plan holds the
+// actual counts).
+struct Level2Bounds {
+    int stride = 0;
+    int64_t nbs_max = 0, fnb = 0, cells = 0;
+};
+
+Level2Bounds level2_bounds(const PartShape& sh, int64_t n, int64_t blocks1) {
+    Level2Bounds b;
+    const int64_t nb1 = 2 * int64_t(sh.p1) - 1;
+    b.stride = sh.two_stage ? sh.p2cap : sh.p2;
+    b.nbs_max = 2 * int64_t(b.stride) - 1;
+    b.fnb = b.nbs_max * nb1;
+    // two-stage: the parts' p2 sum to at most n / target2 + p1, each part
+    // has 2 p2 - 1 buckets, each equality bucket one
+    if (sh.two_stage) b.fnb = std::min(b.fnb, 2 * ((n + sh.target2 - 1) / sh.target2) + nb1);
+    b.cells = b.nbs_max * (blocks1 + nb1);   // every segment rounds its blocks up once
+    return b;
 }
 
 }  // namespace
@@ -548,7 +597,8 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     int rc = tmp.get(&samples, size_t(msort) * esz);
     if (rc == CS_OK) rc = tmp.get(&samples_tmp, size_t(msort) * esz);
     if (rc == CS_OK && sh.two_stage) rc = tmp.get(&smp2, size_t(p1) * size_t(lk.chunk) * esz);
-    if (rc == CS_OK) rc = tmp.get(&gsp, size_t(parts - 1) * size_t(pk.key_size));
+    const int64_t nsplitters = sh.two_stage ? int64_t(p1) * sh.p2cap - 1 : parts - 1;
+    if (rc == CS_OK) rc = tmp.get(&gsp, size_t(nsplitters) * size_t(pk.key_size));
     if (rc == CS_OK && p2 > 1) rc = tmp.get(&l1sp, size_t(std::max(p1 - 1, 1)) * size_t(pk.key_size));
     if (rc == CS_OK) rc = tmp.get(&cor, size_t(ntile_max) + 1);
     if (rc == CS_OK) rc = tmp.get(&work1, size_t(blocks1));
@@ -586,9 +636,10 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
     // two-stage sampling: the second-level splitters from each first-level
     // bucket's own sample (counts1 now holds the bucket starts)
     if (sh.two_stage) {
-        e = pk.l2_gather(values, counts1, blocks1, p1, p2, sh.s2, lk.chunk, n, smp2, stream);
+        e = pk.l2_gather(values, counts1, blocks1, p1, sh.target2, sh.p2cap, sh.s2, lk.chunk, n, smp2, stream);
         if (e == hipSuccess) e = lk.chunk_sort(smp2, smp2, int64_t(p1) * lk.chunk, stream);
-        if (e == hipSuccess) e = pk.l2_pick(smp2, counts1, blocks1, p1, p2, sh.s2, lk.chunk, n, l1sp, gsp, stream);
+        if (e == hipSuccess)
+            e = pk.l2_pick(smp2, counts1, blocks1, p1, sh.target2, sh.p2cap, sh.s2, lk.chunk, n, l1sp, gsp, stream);
         if (e != hipSuccess) return int(e);
     }
 
@@ -613,10 +664,10 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
         // level 2: values -> out, one segment per first-level bucket, laid
         // out on the device; grids are sized by the bounds and the surplus
         // workgroups return at once
-        const int64_t nbs = 2 * int64_t(p2) - 1;
+        const Level2Bounds l2 = level2_bounds(sh, n, blocks1);
         const int64_t blocks2 = blocks1 + nb1;   // every segment rounds up once
-        const int64_t cells2 = nbs * blocks2;
-        fnb = nbs * nb1;
+        const int64_t cells2 = l2.cells;
+        fnb = l2.fnb;
         PartBlock* work2 = nullptr;
         PartSeg* segs2 = nullptr;
         int64_t *hdr2 = nullptr, *counts2 = nullptr, *sums2 = nullptr;
@@ -629,12 +680,14 @@ int partition_sort(int dtype, void* data, int64_t n, void* out, void* values, si
         if (rc == CS_OK) rc = tmp.get(&d_feq, size_t(fnb));
         if (rc != CS_OK) return rc;
         hipLaunchKernelGGL(k_part_level2, dim3(1), dim3(kL2Threads), 0, stream, counts1, blocks1, nb1, n, p1, p2,
-                           target, block_elems, work2, segs2, hdr2);
+                           sh.two_stage ? sh.p2cap : 0, sh.two_stage ? sh.target2 : target, block_elems, work2,
+                           segs2, hdr2);
         e = hipGetLastError();
-        // cells past hdr2[1] are scanned too; no used cell depends on them
-        if (e == hipSuccess) e = pk.count(values, gsp, work2, int(blocks2), p2 - 1, counts2, hdr2, stream);
-        if (e == hipSuccess) e = exclusive_scan(counts2, cells2, sums2, stream);
-        if (e == hipSuccess) e = pk.scatter(values, gsp, work2, int(blocks2), p2 - 1, counts2, out, hdr2, stream);
+        // the scan covers the hdr2[1] cells the level uses
+        if (e == hipSuccess) e = pk.count(values, gsp, work2, int(blocks2), l2.stride - 1, counts2, hdr2, stream);
+        if (e == hipSuccess) e = exclusive_scan(counts2, cells2, sums2, stream, hdr2 + 1);
+        if (e == hipSuccess)
+            e = pk.scatter(values, gsp, work2, int(blocks2), l2.stride - 1, counts2, out, hdr2, stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_part_starts, dim3(unsigned(nb1)), dim3(256), 0, stream, segs2, counts2, d_fst, d_feq,
                                meta, kZero);
@@ -715,7 +768,8 @@ size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_part
                                               r(size_t(sh.p1) * size_t(lk.chunk) * esz)
                                         : 0;
     total += std::max(global_smp, staged_smp);
-    total += r(size_t(parts - 1) * size_t(pk.key_size));                  // global splitters
+    const int64_t nsplit_b = std::max<int64_t>(parts - 1, sh.p2 > 1 ? int64_t(sh.p1) * sh.p2cap - 1 : 0);
+    total += r(size_t(nsplit_b) * size_t(pk.key_size));                   // splitter list
     total += r(size_t(std::max(sh.p1 - 1, 1)) * size_t(pk.key_size));     // first-level splitters
     total += r(size_t(ntile_max + 1) * 8) + r(size_t(nlong_max) * 4);     // long buckets: co-ranks, list
     total += r(size_t(kPlanMeta + 2) * 4);                                // plan counters
@@ -729,9 +783,13 @@ size_t partition_scratch_bytes(int dtype, int64_t n, int min_parts, int max_part
     total += level(1, blocks1, nb1 * blocks1);
     int64_t fnb = nb1;
     if (sh.p2 > 1) {
-        const int64_t nbs = 2 * int64_t(sh.p1) - 1;                       // buckets per second-level segment (bound)
+        // buckets per second-level segment: p2 <= p1 with one global list,
+        // at most p2cap with two-stage sampling (whose parts sum to at most
+        // p1 * p1 + p1)
+        const int64_t nbs = sh.two_stage ? 2 * int64_t(sh.p2cap) - 1 : 2 * int64_t(sh.p1) - 1;
         const int64_t blocks2 = blocks1 + nb1;                            // each segment rounds up once
         fnb = nbs * nb1;
+        if (sh.two_stage) fnb = std::min(fnb, 2 * (int64_t(sh.p1) * sh.p1 + sh.p1) + nb1);
         total += level(nb1, blocks2, nbs * blocks2) + r(4 * 8);           // + device header
     }
     total += r(size_t(fnb + 1) * 8) + r(size_t(fnb));                     // final starts, equality flags

@@ -116,17 +116,27 @@ __global__ void k_part_prep(const typename Elem<DT>::Storage* sorted, int m, int
 
 // Second-level splitters drawn from each first-level bucket (two-stage
 // sampling): first-level part j (bucket 2j, keys strictly between
-// first-level splitters j-1 and j) gets s2 * p2 keys sampled at hashed
-// positions of its own range in the first level's output, sorted as one
-// chunk (the rest of the chunk padded with the maximum key), and every
-// s2-th of them becomes one of its p2 - 1 splitters.  The splitter list
-// keeps the layout of one global list: part j's second-level splitters at
-// j * p2 .. j * p2 + p2 - 2 and first-level splitter j at (j + 1) * p2 - 1,
-// so the second level is laid out and run exactly as before.  A bucket's
-// own sample follows its actual key range, so its final buckets come out at
-// the target size whatever size the first level gave it.
+// first-level splitters j-1 and j) of len keys gets p2 = part_p2(len)
+// second-level parts; s2 * p2 keys sampled at hashed positions of its own
+// range in the first level's output are sorted as one chunk (the rest of
+// the chunk padded with the maximum key), and every s2-th of them becomes
+// one of its p2 - 1 splitters.  The splitter list keeps the layout of one
+// list with a stride of p2cap entries per part: part j's second-level
+// splitters at j * p2cap .., first-level splitter j at (j + 1) * p2cap - 1
+// (the entries between repeat it), so the second level is laid out and run
+// as with one global list.  Each bucket's sample follows its actual size and
+// key range, so its final buckets come out at the target size whatever size
+// the first level gave it.
 // starts: the first level's scanned count matrix (bucket b starts at
 // starts[b * blocks1]).
+// Second-level parts of a first-level part of len keys (two-stage
+// sampling): as many as keep its buckets at `target` or above (so a caller's
+// part bound, folded into target, still holds), at least 1, at most p2cap.
+__host__ __device__ __forceinline__ int part_p2(int64_t len, int64_t target, int p2cap) {
+    const int64_t p = len / target;
+    return int(p < 1 ? 1 : (p > p2cap ? p2cap : p));
+}
+
 template <int DT>
 __device__ __forceinline__ void part_l2_range(const int64_t* starts, int64_t blocks1, int p1, int64_t n, int j,
                                               int64_t& b0, int64_t& len) {
@@ -140,14 +150,15 @@ __device__ __forceinline__ void part_l2_range(const int64_t* starts, int64_t blo
 // is empty), the rest of the chunk the maximum key.  One thread per slot.
 template <int DT>
 __global__ void k_part_l2_gather(const typename Elem<DT>::Storage* src, const int64_t* starts, int64_t blocks1,
-                                 int p1, int p2, int s2, int chunk, int64_t n, typename Elem<DT>::Storage* smp) {
+                                 int p1, int64_t target, int p2cap, int s2, int chunk, int64_t n,
+                                 typename Elem<DT>::Storage* smp) {
     using E = Elem<DT>;
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= int64_t(p1) * chunk) return;
     const int j = int(q / chunk), i = int(q - int64_t(j) * chunk);
     int64_t b0 = 0, len = 0;
     part_l2_range<DT>(starts, blocks1, p1, n, j, b0, len);
-    if (i < s2 * p2 && len > 0) {
+    if (len > 0 && i < s2 * part_p2(len, target, p2cap)) {
         const uint32_t x = uint32_t(i) * 2654435761u ^ uint32_t(j) * 0x9e3779b9u;
         const uint64_t h = (uint64_t(part_hash(x)) << 32) | part_hash(x ^ 0x85ebca6bu);
         smp[q] = src[b0 + int64_t(h % uint64_t(len))];
@@ -156,26 +167,23 @@ __global__ void k_part_l2_gather(const typename Elem<DT>::Storage* src, const in
     }
 }
 
-// gsp[j * p2 + k] (k < p2 - 1) from part j's sorted sample, and
-// gsp[(j + 1) * p2 - 1] = first-level splitter j.  An empty part repeats
-// its lower bound (its segment is carried through unsplit and never reads
-// them).  One thread per splitter.
+// gsp[j * p2cap + k]: part j's second-level splitters (k < p2 - 1) from its
+// sorted sample, then first-level splitter j repeated up to (j + 1) * p2cap
+// - 1 (the last part: the maximum key), so the list never decreases.  One
+// thread per entry.
 template <int DT>
 __global__ void k_part_l2_pick(const typename Elem<DT>::Storage* smp, const int64_t* starts, int64_t blocks1, int p1,
-                               int p2, int s2, int chunk, int64_t n, const typename Elem<DT>::Key* l1sp,
-                               typename Elem<DT>::Key* gsp) {
+                               int64_t target, int p2cap, int s2, int chunk, int64_t n,
+                               const typename Elem<DT>::Key* l1sp, typename Elem<DT>::Key* gsp) {
     using E = Elem<DT>;
     const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= int64_t(p1) * p2 - 1) return;
-    const int j = int(q / p2), k = int(q - int64_t(j) * p2);
-    if (k == p2 - 1) {
-        gsp[q] = l1sp[j];
-        return;
-    }
+    if (q >= int64_t(p1) * p2cap - 1) return;
+    const int j = int(q / p2cap), k = int(q - int64_t(j) * p2cap);
     int64_t b0 = 0, len = 0;
     part_l2_range<DT>(starts, blocks1, p1, n, j, b0, len);
-    if (len > 0) gsp[q] = E::key(smp[int64_t(j) * chunk + int64_t(k + 1) * s2 - 1]);
-    else gsp[q] = j > 0 ? l1sp[j - 1] : l1sp[0];
+    const int p2 = len > 0 ? part_p2(len, target, p2cap) : 1;
+    if (k < p2 - 1) gsp[q] = E::key(smp[int64_t(j) * chunk + int64_t(k + 1) * s2 - 1]);
+    else gsp[q] = j < p1 - 1 ? l1sp[j] : E::kPad;
 }
 
 // Classification.  A lookup table indexed by kPartLutBits bits of the key's
@@ -625,13 +633,14 @@ __global__ void k_part_starts(const PartSeg* segs, const int64_t* offsets, int64
 
 // Exclusive scan of int64 values in place, three launches: per-block sums,
 // a single-block scan of those sums, and the per-block scan with its offset.
+// With d_n the count is *d_n (at most n, known on the device).
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__global__ void k_scan_sums(const int64_t* v, int64_t n, int64_t* sums);
+__global__ void k_scan_sums(const int64_t* v, int64_t n, int64_t* sums, const int64_t* d_n);
 __global__ void k_scan_top(int64_t* sums, int64_t nblocks);
-__global__ void k_scan_apply(int64_t* v, int64_t n, const int64_t* sums);
+__global__ void k_scan_apply(int64_t* v, int64_t n, const int64_t* sums, const int64_t* d_n);
 
 // Merge pass over a list of oversized buckets: item = (bucket base, bucket
 // length, first output position of this tile inside the bucket), plus the
@@ -863,10 +872,10 @@ struct PartKernels {
                        int64_t block_elems, int64_t blocks1, PartBlock* work1, PartSeg* segs1,
                        hipStream_t s) = nullptr;
     // k_part_l2_gather + k_part_l2_pick around the caller's chunk sort
-    hipError_t (*l2_gather)(const void* src, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2,
-                            int chunk, int64_t n, void* smp, hipStream_t s) = nullptr;
-    hipError_t (*l2_pick)(const void* smp, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2, int chunk,
-                          int64_t n, const void* l1sp, void* gsp, hipStream_t s) = nullptr;
+    hipError_t (*l2_gather)(const void* src, const int64_t* starts, int64_t blocks1, int p1, int64_t target,
+                            int p2cap, int s2, int chunk, int64_t n, void* smp, hipStream_t s) = nullptr;
+    hipError_t (*l2_pick)(const void* smp, const int64_t* starts, int64_t blocks1, int p1, int64_t target, int p2cap,
+                          int s2, int chunk, int64_t n, const void* l1sp, void* gsp, hipStream_t s) = nullptr;
     // count / scatter over `blocks` work items; with nwork (device) only the
     // first *nwork of them exist and the rest of the grid returns at once.
     hipError_t (*count)(const void* src, const void* splitters, const PartBlock* work, int blocks, int max_ns,
@@ -909,18 +918,19 @@ PartKernels part_kernels() {
                            static_cast<K*>(l1sp), n, block_elems, blocks1, work1, segs1);
         return hipGetLastError();
     };
-    k.l2_gather = [](const void* src, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2, int chunk,
-                     int64_t n, void* smp, hipStream_t s) {
+    k.l2_gather = [](const void* src, const int64_t* starts, int64_t blocks1, int p1, int64_t target, int p2cap,
+                     int s2, int chunk, int64_t n, void* smp, hipStream_t s) {
         const int64_t threads = int64_t(p1) * chunk;
         hipLaunchKernelGGL(k_part_l2_gather<DT>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
-                           static_cast<const S*>(src), starts, blocks1, p1, p2, s2, chunk, n, static_cast<S*>(smp));
+                           static_cast<const S*>(src), starts, blocks1, p1, target, p2cap, s2, chunk, n,
+                           static_cast<S*>(smp));
         return hipGetLastError();
     };
-    k.l2_pick = [](const void* smp, const int64_t* starts, int64_t blocks1, int p1, int p2, int s2, int chunk,
-                   int64_t n, const void* l1sp, void* gsp, hipStream_t s) {
-        const int64_t threads = int64_t(p1) * p2;
+    k.l2_pick = [](const void* smp, const int64_t* starts, int64_t blocks1, int p1, int64_t target, int p2cap, int s2,
+                   int chunk, int64_t n, const void* l1sp, void* gsp, hipStream_t s) {
+        const int64_t threads = int64_t(p1) * p2cap;
         hipLaunchKernelGGL(k_part_l2_pick<DT>, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
-                           static_cast<const S*>(smp), starts, blocks1, p1, p2, s2, chunk, n,
+                           static_cast<const S*>(smp), starts, blocks1, p1, target, p2cap, s2, chunk, n,
                            static_cast<const K*>(l1sp), static_cast<K*>(gsp));
         return hipGetLastError();
     };
