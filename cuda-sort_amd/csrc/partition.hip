@@ -487,7 +487,9 @@ PartShape part_shape(int dtype, int64_t n, int min_parts, int max_parts) {
     // elements) above which a bucket takes the long path.  2^28 int32 (ms
     // per sort, 16 samples per part): 1400 3.80, 1536 3.76, 1700 3.91,
     // 3072 4.13; the 2048-key tile sorts a key in fewer layers than the
-    // 4096-key one, which outweighs twice the parts.
+    // 4096-key one, which outweighs twice the parts.  With split buckets and
+    // two-stage sampling the optimum is flat: 1280 3.42, 1408 3.40, 1536
+    // 3.38-3.39, 1664 3.40, 1792 3.40, 2048 3.42.
     sh.target = 1536;
     int64_t P = (n + sh.target - 1) / sh.target;
     if (min_parts > 0) P = std::max<int64_t>(P, min_parts);
