@@ -17,10 +17,13 @@
 //     each tile in bucket order in LDS (ranks from LDS atomics; order inside
 //     a bucket is free, the sort is unstable like the reference's) and
 //     writes each bucket's run of the tile to consecutive addresses.
-// The host then reads the bucket sizes once (the reference also reads its
-// counters back, cusort.hpp:1967-1977) and sorts the buckets as one ragged
-// batch; buckets longer than the on-chip tile are cut into chunks whose
-// sorted runs are merged by tiled merge passes over a tile list.
+// Two levels run steps 2-5 twice, the second once per first-level bucket,
+// with that bucket's splitters drawn from its own sample.  The final
+// buckets are then sorted as one ragged batch planned on the device (the
+// reference reads its counters back to the host instead, cusort.hpp:
+// 1967-1977): buckets just over the single-wave tile as head and tail
+// merged afterwards, buckets longer than a chunk by chunk sorts and merge
+// passes.  Nothing is read back to the host.
 #pragma once
 
 #include <algorithm>
