@@ -17,12 +17,23 @@
 // slower on gfx950: 4096 x 1024 int32 at 160 / 192 / 224 threads took 89 /
 // 77 / 84 us against 82 / 69 / 75 us with a barrier after every layer.
 //
+// ASYNC (the reference's cooperative_groups::memcpy_async staging,
+// cusort.hpp:808-837, 1021-1046): for key-only element types the array's
+// bytes go from global memory straight into LDS by direct loads
+// (global_load_lds_dword, one dword per lane, no VGPR round trip), and
+// floating-point keys are then turned into their order-preserving integer
+// keys in place.  Records keep the staged loads: their LDS layout splits the
+// keys from the payloads, which a byte copy cannot do.  The store back goes
+// through VGPRs either way (gfx950 has no LDS-to-global transfer).
+//
 // TESTSORTED: before the network and after every stage k >= 5 the block
 // checks whether the array is already sorted and stops if so.  A sorted
 // array is a fixed point of every remaining layer (no strict inversion ever
 // exists), so the output is the same with or without the checks
 // (cusort.hpp:394-458 checks at the same kind of checkpoints).
 #pragma once
+
+#include <type_traits>
 
 #include "elem.hpp"
 
@@ -71,14 +82,44 @@ k_bitonic_lds(const typename E::Storage* in, typename E::Storage* out, int64_t a
     const int T = int(blockDim.x);
     const int tid = int(threadIdx.x);
 
-    for (int i = tid; i < W; i += T) {
-        if (i < n) {
-            const typename E::Storage s = src[i];
-            lk[i] = E::key(s);
-            if constexpr (E::kPayload) lp[i] = E::payload(s);
-        } else {
-            lk[i] = E::kPad;
-            if constexpr (E::kPayload) lp[i] = INT32_MAX;
+    bool staged = true;
+    if constexpr (!E::kPayload && sizeof(typename E::Storage) == sizeof(Key)) {
+        if (flags & CS_FLAG_ASYNC) {   // workgroup-uniform
+            staged = false;
+            // dword d of the array to LDS dword d: each wave's lanes cover
+            // consecutive dwords, so the wave's LDS base (its first dword)
+            // is uniform, as the instruction requires
+            const int dwords = int(size_t(n) * sizeof(Key) / 4);
+            const uint32_t* gsrc = reinterpret_cast<const uint32_t*>(src);
+            uint32_t* ldw = reinterpret_cast<uint32_t*>(lk);
+            const int lane = tid & 63;
+            for (int d = tid; d < dwords; d += T)
+                __builtin_amdgcn_global_load_lds(gsrc + d, (__attribute__((address_space(3))) void*)(ldw + (d - lane)),
+                                                 4, 0, 0);
+            for (int i = n + tid; i < W; i += T) lk[i] = E::kPad;
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            // keys that are not their own storage bits (fp totalOrder) are
+            // converted in place, each thread its own positions
+            if constexpr (std::is_same<E, Elem<CS_F32>>::value || std::is_same<E, Elem<CS_F64>>::value) {
+                for (int i = tid; i < n; i += T) {
+                    typename E::Storage raw;
+                    __builtin_memcpy(&raw, &lk[i], sizeof(raw));
+                    lk[i] = E::key(raw);
+                }
+            }
+        }
+    }
+    if (staged) {
+        for (int i = tid; i < W; i += T) {
+            if (i < n) {
+                const typename E::Storage s = src[i];
+                lk[i] = E::key(s);
+                if constexpr (E::kPayload) lp[i] = E::payload(s);
+            } else {
+                lk[i] = E::kPad;
+                if constexpr (E::kPayload) lp[i] = INT32_MAX;
+            }
         }
     }
     __syncthreads();

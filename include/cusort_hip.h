@@ -69,7 +69,8 @@ enum cs_method {
  * TESTSORTED  skip the network when the array is already sorted (output
  *             unchanged, only time differs; cusort.hpp:394-458)
  * FULLSM      exchange every layer through LDS instead of registers
- * ASYNC       stage the global load through LDS (memcpy_async analogue)   */
+ * ASYNC       load key-only arrays of the LDS network straight into LDS
+ *             (global_load_lds; memcpy_async analogue, cusort.hpp:808-837) */
 #define CS_FLAG_TESTSORTED 1u
 #define CS_FLAG_FULLSM 2u
 #define CS_FLAG_ASYNC 4u

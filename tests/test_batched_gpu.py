@@ -106,6 +106,29 @@ def test_every_method_and_thread_count(cs, orc, gpu, dtype, n):
                 run_case(cs, orc, gpu, dt, dist, n, 7, method, threads, flags)
 
 
+@pytest.mark.parametrize("dtype", ["I32", "U32", "F32", "I64", "U64", "F64", "REC_I32_I32", "REC_I64_I32"])
+@pytest.mark.parametrize("n", [32, 33, 100, 255, 1000, 1024])
+def test_async_direct_lds_loads(cs, orc, gpu, dtype, n):
+    """ASYNC on the LDS network (v1, v2, and v3 / v4 at thread counts that
+    are not a power of two): key-only arrays reach LDS by direct
+    global-to-LDS loads, fp keys are converted in place, records keep the
+    staged loads; every case against the oracle, including odd lengths,
+    partial last waves and fp special values."""
+    dt = getattr(cs, dtype)
+    w = max(32, 1 << (n - 1).bit_length())
+    dists = [orc.D_UNIFORM, orc.D_DUP50, orc.D_SORTED, orc.D_EXTREMES6]
+    if dt in (cs.F32, cs.F64):
+        dists.append(orc.D_NORMAL_SPECIAL)
+    cases = [(cs.V1, w), (cs.V2, w), (cs.V3, 96), (cs.V4, 160)]
+    for method, threads in cases:
+        if threads > w:
+            continue
+        for flags in (cs.FLAG_ASYNC, cs.FLAG_ASYNC | cs.FLAG_TESTSORTED):
+            for dist in dists:
+                run_case(cs, orc, gpu, dt, dist, n, 9, method, threads, flags, seed=n + dist)
+        run_case(cs, orc, gpu, dt, orc.D_UNIFORM, n, 5, method, threads, cs.FLAG_ASYNC, inplace=True, offset_elems=1)
+
+
 @pytest.mark.parametrize("dtype", ["I32", "F32", "REC_I32_I32", "U64", "REC_I64_I32"])
 @pytest.mark.parametrize("n", [32, 100, 1024, 2048, 4096])
 def test_testsorted_mixed_arrays(cs, orc, gpu, dtype, n):
