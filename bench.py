@@ -88,6 +88,35 @@ def warm_up(step, count, min_seconds=0.1):
     return done
 
 
+def time_launches(step, steps, stream, hold_us=200.0):
+    """Milliseconds per step over `steps` calls of `step` on `stream`, from HIP
+    events recorded on that stream.  A short bounded hold kernel
+    (cs_stream_hold) keeps the stream busy while the host enqueues the first
+    launches, so that the run starts with launches already queued: a ctypes
+    call and launch take ~10 us, close to one 4096x1024 sort, and without the
+    hold the first launch of a short run lands after a host gap.  The hold is
+    kept short because an idle GPU lowers its clock within milliseconds
+    (measured on MI355X: a 17 ms hold before 400 sorts made them 10 %
+    slower).  Warns when the host enqueued more slowly than the device ran,
+    in which case host gaps may be part of the time."""
+    import cusort_hip as cs
+    import hiprt
+    cs.stream_hold(hold_us, stream.handle)
+    start, end = hiprt.Event(), hiprt.Event()
+    start.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    end.record(stream)
+    enqueue_us = (time.perf_counter() - t0) * 1e6
+    end.synchronize()
+    ms = start.elapsed_ms(end) / steps
+    if enqueue_us > hold_us + ms * 1e3 * steps:
+        print(f"bench: enqueueing {steps} steps took {enqueue_us:.0f} us, longer than the hold and the device time "
+              f"({hold_us:.0f} + {ms * 1e3 * steps:.0f} us): host gaps may be part of the time", file=sys.stderr)
+    return ms
+
+
 def bench_batched(args, dtype_name: str):
     import cusort_hip as cs
     import hiprt
@@ -109,14 +138,8 @@ def bench_batched(args, dtype_name: str):
 
     warm_up(step, args.warmup)
     # HIP events on the stream the kernels run on bracket exactly K launches
-    start, end = hiprt.Event(), hiprt.Event()
-    start.record(stream)
-    for _ in range(args.steps):
-        step()
-    end.record(stream)
-    end.synchronize()
+    ms = time_launches(step, args.steps, stream)
     hiprt.synchronize()
-    ms = start.elapsed_ms(end) / args.steps
 
     # correctness of the measured output (outside the timed region)
     res = cs.to_numpy(dst, dt, count)
@@ -187,13 +210,7 @@ def bench_pairs(args):
 
     def timed(step):
         warm_up(step, args.warmup)
-        start, end = hiprt.Event(), hiprt.Event()
-        start.record(stream)
-        for _ in range(args.steps):
-            step()
-        end.record(stream)
-        end.synchronize()
-        return start.elapsed_ms(end) / args.steps
+        return time_launches(step, args.steps, stream)
 
     ms = timed(lambda: cs.batched_sort_pairs(cs.I64, kin.ptr, vin.ptr, kout.ptr, vout.ptr, arrays, n,
                                              stream.handle))
@@ -237,14 +254,9 @@ def bench_copy(args):
     nbytes = args.copy_mib << 20
     src, dst = hiprt.Buffer(nbytes), hiprt.Buffer(nbytes)
     src.fill(1)
-    warm_up(lambda: cs.device_copy(dst.ptr, src.ptr, nbytes, stream.handle), args.warmup)
-    start, end = hiprt.Event(), hiprt.Event()
-    start.record(stream)
-    for _ in range(args.steps):
-        cs.device_copy(dst.ptr, src.ptr, nbytes, stream.handle)
-    end.record(stream)
-    end.synchronize()
-    ms = start.elapsed_ms(end) / args.steps
+    copy = lambda: cs.device_copy(dst.ptr, src.ptr, nbytes, stream.handle)
+    warm_up(copy, args.warmup)
+    ms = time_launches(copy, args.steps, stream)
     gbs = 2 * nbytes / (ms * 1e-3) / 1e9
     return {"metric": "device copy bandwidth (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
@@ -276,13 +288,7 @@ def bench_ragged(args, dtype_name: str):
 
     def timed(fn):
         warm_up(fn, args.warmup)
-        start, end = hiprt.Event(), hiprt.Event()
-        start.record(stream)
-        for _ in range(args.steps):
-            fn()
-        end.record(stream)
-        end.synchronize()
-        return start.elapsed_ms(end) / args.steps
+        return time_launches(fn, args.steps, stream)
 
     ms_p = timed(lambda: plan.sort(src.ptr, dst.ptr, doff.ptr, stream.handle))
     res_p = cs.to_numpy(dst, dt, total)
@@ -354,12 +360,7 @@ def bench_large(args, dtype_name: str):
     total = 0.0
     for _ in range(args.steps):
         cs.device_copy(work.ptr, pristine.ptr, nbytes, stream.handle)
-        start, end = hiprt.Event(), hiprt.Event()
-        start.record(stream)
-        sort()
-        end.record(stream)
-        end.synchronize()
-        total += start.elapsed_ms(end)
+        total += time_launches(sort, 1, stream, hold_us=300.0)
     ms = total / args.steps
     ok, cpu = None, None
     if not args.no_cpu_baseline:

@@ -269,6 +269,12 @@ int cs_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream)
     return e == hipSuccess ? CS_OK : int(e);
 }
 
+int cs_stream_hold(double microseconds, hipStream_t stream) {
+    if (!(microseconds >= 0.0 && microseconds <= 1e6)) return CS_E_INVALID_VALUE;
+    const hipError_t e = cs::stream_hold(microseconds, stream);
+    return e == hipSuccess ? CS_OK : int(e);
+}
+
 int cs_generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, void* out, hipStream_t stream) {
     if (!valid_dtype(dtype) || dist < CS_DIST_UNIFORM || dist > CS_DIST_PRESORTED_RUNS) return CS_E_INVALID_VALUE;
     if (n < 1 || count < 0 || (count > 0 && !out)) return CS_E_INVALID_VALUE;

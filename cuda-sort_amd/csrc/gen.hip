@@ -167,7 +167,25 @@ __global__ void __launch_bounds__(256) k_copy1(const unsigned char* __restrict__
         dst[i] = src[i];
 }
 
+// Occupies its stream for `ticks` periods of the constant wall clock, then
+// exits: a bounded wait (one wave, sleeping between clock reads).
+__global__ void __launch_bounds__(64) k_stream_hold(uint64_t ticks) {
+    const uint64_t t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(32);
+}
+
 }  // namespace
+
+hipError_t stream_hold(double microseconds, hipStream_t s) {
+    int dev = 0, khz = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (e != hipSuccess) return e;
+    if (khz <= 0) return hipErrorInvalidValue;
+    const uint64_t ticks = uint64_t(microseconds * double(khz) / 1000.0);
+    hipLaunchKernelGGL(k_stream_hold, dim3(1), dim3(64), 0, s, ticks);
+    return hipGetLastError();
+}
 
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (bytes == 0) return hipSuccess;

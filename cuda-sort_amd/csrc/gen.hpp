@@ -5,6 +5,7 @@
 
 namespace cs {
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+hipError_t stream_hold(double microseconds, hipStream_t s);
 hipError_t generate(int dtype, int dist, int64_t n, int64_t count, uint64_t seed, void* out, hipStream_t s);
 // Stream-ordered allocation from a per-device pool that keeps freed memory
 // cached (unlimited release threshold), so that repeated large sorts do not

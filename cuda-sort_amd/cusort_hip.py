@@ -152,6 +152,8 @@ def lib():
         L.cs_server_pool_in_use.argtypes = [vp, P64]
         L.cs_device_copy.restype = ctypes.c_int
         L.cs_device_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.cs_stream_hold.restype = ctypes.c_int
+        L.cs_stream_hold.argtypes = [ctypes.c_double, vp]
         L.cs_generate.restype = ctypes.c_int
         L.cs_generate.argtypes = [ctypes.c_int, ctypes.c_int, i64, i64, u64, vp, vp]
         _lib = L
@@ -212,6 +214,10 @@ def generate(dtype: int, dist: int, n: int, count: int, seed: int, dst: int, str
 
 def device_copy(dst: int, src: int, nbytes: int, stream: int = 0) -> None:
     check(lib().cs_device_copy(dst, src, nbytes, stream), "cs_device_copy")
+
+
+def stream_hold(microseconds: float, stream: int = 0) -> None:
+    check(lib().cs_stream_hold(float(microseconds), stream), "cs_stream_hold")
 
 
 SEG_NO_GRAPH = 1

@@ -355,6 +355,14 @@ int cs_server_pool_in_use(cs_server* s, int64_t* bytes);
  * traffic is known exactly, to calibrate rocprofv3's FETCH_SIZE/WRITE_SIZE. */
 int cs_device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
+/* Enqueues one small kernel that keeps `stream` busy for about
+ * `microseconds` (0 .. 1e6; a bounded wait on the device's constant wall
+ * clock, then it exits).  No counterpart in the reference: bench.cu brackets
+ * its launches with events directly (bench.cu:337-348).  Work queued behind
+ * it starts back to back once it ends, so events around a run of short
+ * launches time the device, not the host's launch rate. */
+int cs_stream_hold(double microseconds, hipStream_t stream);
+
 /* Input distributions of cs_generate (numbering shared with the oracle). */
 enum cs_dist {
     CS_DIST_UNIFORM = 0,        /* uniform over the full key range            */

@@ -203,3 +203,19 @@ def test_describe_reports_resources(cs, gpu):
     assert 0 < d["vgprs"] <= 128
     d = cs.describe(cs.I32, 1000, cs.V3, 96)
     assert d["block_threads"] == 96 and d["arrays_per_block"] == 1
+
+
+def test_stream_hold_delays_queued_work(cs, gpu):
+    """cs_stream_hold keeps its stream busy for about the time asked (the
+    benchmark queues its timed launches behind it), then exits."""
+    stream = gpu.Stream()
+    start, mid, end = gpu.Event(), gpu.Event(), gpu.Event()
+    start.record(stream)
+    cs.stream_hold(2000.0, stream.handle)
+    mid.record(stream)
+    cs.stream_hold(0.0, stream.handle)
+    end.record(stream)
+    end.synchronize()
+    held = start.elapsed_ms(mid)
+    assert 1.9 <= held <= 50.0, held
+    assert mid.elapsed_ms(end) < 1.0

@@ -66,6 +66,12 @@ def test_domain_rejections(cs, args, want):
     assert cs.batched_status(dt, FAKE, FAKE, arrays, n, method, threads, flags) == want
 
 
+def test_stream_hold_rejects_out_of_range(cs):
+    L = cs.lib()
+    for us in (-1.0, 1e6 + 1, float("nan")):
+        assert L.cs_stream_hold(us, None) == cs.E_INVALID_VALUE
+
+
 def test_null_and_overlap(cs):
     assert cs.batched_status(0, 0, FAKE, 4, 64) == cs.E_INVALID_VALUE
     # partially overlapping in/out ranges are rejected; identical = in place
