@@ -82,6 +82,14 @@ struct Tile {
     static constexpr int e = LOGE;
     static constexpr int w = LOGW;
 
+    // Elements whose compare-exchange is a compare and selects (records,
+    // 8-byte keys) open with an odd-even merge sort of each lane's slots:
+    // 191 compare-exchanges for 32 slots against the bitonic stages' 240.
+    // 4-byte keys (one v_min + one v_max per pair) keep the bitonic stages,
+    // whose layers carry more independent pairs.
+    static constexpr bool kLaneSort = E::kPayload || sizeof(Key) > 4;
+    using P = PlanOf<LOGW, LOGE, kLaneSort>;
+
     Key k[NE];
     int32_t p[NE];
 
@@ -90,9 +98,14 @@ struct Tile {
 
     template <int I>
     __device__ __forceinline__ void run(int t, Key* lk, int32_t* lp) {
-        if constexpr (I < PlanOf<LOGW, LOGE>::value.nops) {
-            constexpr Op op = PlanOf<LOGW, LOGE>::value.ops[I];
-            if constexpr (op.kind == OP_REFLECT) {
+        if constexpr (I < P::value.nops) {
+            constexpr Op op = P::value.ops[I];
+            if constexpr (op.kind == OP_LANESORT) {
+                sfor<0, OemOf<NE>::value.n>([&](auto i_) {
+                    constexpr int i = decltype(i_)::value;
+                    cx<false>(OemOf<NE>::value.c[i].a, OemOf<NE>::value.c[i].b);
+                });
+            } else if constexpr (op.kind == OP_REFLECT) {
                 constexpr int mask = (2 << op.bit) - 1;
                 sfor<0, NE>([&](auto r_) {
                     constexpr int r = decltype(r_)::value;

@@ -9,11 +9,16 @@
 #define CS_HD
 #endif
 
+#include "xlane_plan.hpp"
+
 namespace cs {
 
 // ------------------------------------------------------------------ plan --
 constexpr int kMaxOps = 320;
-enum : int { OP_REFLECT = 0, OP_CE = 1, OP_TRANSPOSE = 2 };
+// OP_LANESORT: every lane sorts its E slots ascending (Batcher's odd-even
+// merge sort, xl::make_oem(E)); it stands for stages 1..e, whose only
+// contract towards stage e+1 is ascending runs of E positions.
+enum : int { OP_REFLECT = 0, OP_CE = 1, OP_TRANSPOSE = 2, OP_LANESORT = 3 };
 
 struct Op {
     int kind = 0;
@@ -33,11 +38,16 @@ struct Plan {
     int ntrans = 0;
 };
 
-constexpr Plan make_plan(int w, int e) {
+constexpr Plan make_plan(int w, int e, bool lanesort = false) {
     Plan p{};
     if (e <= 0) return p;        // E = 1 has no slot bits: not a register tile
     int cur = e - 1;
-    for (int k = 1; k <= w; ++k) {
+    if (lanesort) {
+        Op l{};
+        l.kind = OP_LANESORT; l.cur_h = cur;
+        p.ops[p.nops++] = l;
+    }
+    for (int k = lanesort ? e + 1 : 1; k <= w; ++k) {
         if (k <= e) {
             Op r{};
             r.kind = OP_REFLECT; r.bit = k - 1; r.cur_h = cur;
@@ -74,8 +84,11 @@ constexpr Plan make_plan(int w, int e) {
     return p;
 }
 
-template <int LOGW, int LOGE> struct PlanOf {
-    static constexpr Plan value = make_plan(LOGW, LOGE);
+template <int LOGW, int LOGE, bool LANESORT = false> struct PlanOf {
+    static constexpr Plan value = make_plan(LOGW, LOGE, LANESORT);
+};
+template <int N> struct OemOf {
+    static constexpr xl::CxList value = xl::make_oem(N);
 };
 
 // Logical index of slot 0 of lane t in layout L(h) (e slot bits): the low

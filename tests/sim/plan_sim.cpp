@@ -1,6 +1,8 @@
 // Host execution of the register-tile bitonic schedule (cuda-sort_amd/csrc/plan.hpp),
 // lane by lane and slot by slot, with the same layout, flip and LDS-address
-// helpers the kernel uses.  It checks, for a (w, e) tiling:
+// helpers the kernel uses, with or without the in-lane odd-even opening
+// (third argument 1: OP_LANESORT, the form records and 8-byte keys run).
+// It checks, for a (w, e) tiling:
 //   1. the 0-1 principle: every one of the 2^W binary inputs comes out
 //      sorted (W <= 16), else many random binary inputs;
 //   2. random integer inputs with many duplicates come out sorted and are a
@@ -26,7 +28,8 @@ struct Sim {
     Plan plan;
     std::vector<int64_t> k, p;   // [t * E + r]
     std::vector<int64_t> lk, lp;
-    Sim(int w_, int e_) : w(w_), e(e_), T(1 << (w_ - e_)), E(1 << e_), W(1 << w_), plan(make_plan(w_, e_)) {
+    Sim(int w_, int e_, bool lanesort)
+        : w(w_), e(e_), T(1 << (w_ - e_)), E(1 << e_), W(1 << w_), plan(make_plan(w_, e_, lanesort)) {
         lk.resize(size_t(W) * 2 + 64);
         lp.resize(lk.size());
     }
@@ -42,7 +45,11 @@ struct Sim {
         p.assign(pay.begin(), pay.end());
         for (int i = 0; i < plan.nops; ++i) {
             const Op& op = plan.ops[i];
-            if (op.kind == OP_REFLECT) {
+            if (op.kind == OP_LANESORT) {
+                const xl::CxList l = xl::make_oem(E);
+                for (int t = 0; t < T; ++t)
+                    for (int c = 0; c < l.n; ++c) cx(t, l.c[c].a, l.c[c].b, false);
+            } else if (op.kind == OP_REFLECT) {
                 const int mask = (2 << op.bit) - 1;
                 for (int t = 0; t < T; ++t)
                     for (int r = 0; r < E; ++r)
@@ -88,9 +95,9 @@ struct Sim {
 };
 
 int main(int argc, char** argv) {
-    if (argc < 3) { std::printf("usage: plan_sim w e\n"); return 2; }
+    if (argc < 3) { std::printf("usage: plan_sim w e [lanesort]\n"); return 2; }
     const int w = std::atoi(argv[1]), e = std::atoi(argv[2]);
-    Sim sim(w, e);
+    Sim sim(w, e, argc > 3 && std::atoi(argv[3]) != 0);
     const int W = sim.W;
     std::vector<int64_t> keys(static_cast<size_t>(W)), pay(static_cast<size_t>(W));
     // 1. zero-one principle

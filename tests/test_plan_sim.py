@@ -23,7 +23,8 @@ def sim(tmp_path_factory):
 CASES = [(w, e) for w in range(1, 15) for e in range(1, 6) if e <= w and w - e <= 10]
 
 
+@pytest.mark.parametrize("lanesort", [0, 1])
 @pytest.mark.parametrize("w,e", CASES)
-def test_schedule_sorts(sim, w, e):
-    r = subprocess.run([sim, str(w), str(e)], capture_output=True, text=True, timeout=300)
+def test_schedule_sorts(sim, w, e, lanesort):
+    r = subprocess.run([sim, str(w), str(e), str(lanesort)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
