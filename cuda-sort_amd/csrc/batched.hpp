@@ -71,7 +71,7 @@ constexpr bool tile_exists(int logw, int loge) {
     return (DT == CS_I32 || DT == CS_REC_I32_I32) && loge <= 4;
 }
 
-// Key-only 4-byte element types sort 32..1024-element arrays with the
+// Key-only 4-byte element types sort 32..2048-element arrays with the
 // cross-lane register engine (bitonic_xlane.hpp) instead of the LDS-transpose
 // tile of the same shape (32 keys per lane, W / 32 lanes per array).  Stages
 // from kXlaneWindow on move through LDS windows: at W = 1024 that keeps lane
@@ -80,7 +80,7 @@ constexpr bool tile_exists(int logw, int loge) {
 constexpr int kXlaneWindow = 9;
 constexpr bool xlane_dtype(int dt) { return dt == CS_I32 || dt == CS_U32 || dt == CS_F32; }
 constexpr bool xlane_shape(int dt, int logw, int loge) {
-    return xlane_dtype(dt) && loge == xl::kLogE && logw >= xl::kLogE && logw <= 10;
+    return xlane_dtype(dt) && loge == xl::kLogE && logw >= xl::kLogE && logw <= 11;
 }
 
 template <int DT, int LW>

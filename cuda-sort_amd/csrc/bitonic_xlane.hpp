@@ -1,6 +1,6 @@
 // Cross-lane register bitonic sort for 4-byte keys without payload (int32,
 // uint32, fp32 through its totalOrder key), W = 2^LOGW positions per array,
-// 32 <= W <= 1024: every array is held by T = W / 32 lanes of one wave with
+// 32 <= W <= 2048: every array is held by T = W / 32 lanes of one wave with
 // 32 keys per lane in VGPRs (64 / T arrays per wave).
 //
 // Network: an odd-even merge sorter on each lane's 32 registers (191
@@ -94,7 +94,10 @@ template <class E, int LOGW, int LDSK>
 struct XlTile {
     using K = typename E::Key;
     static_assert(!E::kPayload && sizeof(K) == 4, "cross-lane engine: 4-byte keys without payload");
-    static_assert(LOGW >= 5 && LOGW <= 10, "cross-lane engine: 32..1024 positions per array");
+    static_assert(LOGW >= 5 && LOGW <= 11, "cross-lane engine: 32..2048 positions per array");
+    // With 64 lanes per array the sixth lane bit sits at position 2 (lane ^ 4,
+    // no single DPP move): only a window stage may touch it.
+    static_assert(LOGW <= 10 || LDSK <= 11, "2048 positions need the window for stage 11");
     static constexpr int LOGT = LOGW - xl::kLogE;
     static constexpr int T = 1 << LOGT;
     static constexpr K kLo = KeyLimits<K>::lo;
@@ -227,7 +230,9 @@ struct XlTile {
                 wave_lds_fence();
                 get_std(lk, t);
                 wave_lds_fence();
-                reg_layers_down<KS - 6, last>(cl);   // logical bits KS-6 .. 0
+                // logical bits KS-6 .. 5 (lane bits), then 4 .. 0
+                sfor<0, (KS > 10 ? KS - 10 : 0)>([&](auto i_) { lane_layer<KS - 11 - decltype(i_)::value>(cl, t); });
+                reg_layers_down<(KS - 6 < 4 ? KS - 6 : 4), last>(cl);
             }
             stages<KS + 1>(t, lk);
         }

@@ -16,6 +16,8 @@
 //   variant 0: register tile, 32 keys per lane (16 for 16-byte records)
 //   variant 1: register tile, 16 keys per lane
 //   variant 2: LDS-resident network, one segment per workgroup
+//   variant 3: cross-lane register engine (bitonic_xlane.hpp), 4-byte keys
+//              without payload, W = 32..2048
 // Every wave reaches the loop exit once the counter passes the item count.
 //
 // The reference's only ragged sort is SortPartitions (cusort.hpp:1802-1846):
@@ -29,11 +31,11 @@
 namespace cs {
 
 constexpr int kSegBlock = 256;      // threads per workgroup of the segmented kernels
-constexpr int kSegVariants = 3;
+constexpr int kSegVariants = 4;
 
 struct SegItem {
     int bucket;    // log2 W
-    int variant;   // 0, 1, 2 (see above)
+    int variant;   // 0 .. 3 (see above)
     int first;     // index into the bucket-ordered segment list
     int count;     // segments in this item
 };
@@ -53,6 +55,7 @@ template <int DT> constexpr int seg_loge(int b, int variant) {
 template <int DT> constexpr bool seg_variant_ok(int b, int variant) {
     if (b < 0 || b > seg_max_logw<DT>()) return false;
     if (variant == 2) return b >= 1;
+    if (variant == 3) return xlane_dtype(DT) && b >= xl::kLogE && b <= 11;
     const int le = seg_loge<DT>(b, variant);
     if (le < 0 || (variant == 1 && b < 5)) return false;
     if (b > 0 && le == 0) return false;
@@ -61,6 +64,7 @@ template <int DT> constexpr bool seg_variant_ok(int b, int variant) {
 
 template <int DT> constexpr int seg_segments_per_item(int b, int variant) {
     if (variant == 2) return 1;
+    if (variant == 3) return kSegBlock >> (b - xl::kLogE);
     const int T = 1 << (b - seg_loge<DT>(b, variant));
     return kSegBlock / T;
 }
@@ -86,12 +90,17 @@ template <int DT> constexpr int seg_small_max() {
 // kernel for buckets BLO..BHI needs: the maximum over their variants.
 template <int DT, int BLO, int BHI> constexpr int seg_lds_words() {
     int m = 1 << BHI;    // LDS-resident variant: one segment of W
-    for (int b = BLO; b <= BHI; ++b)
+    for (int b = BLO; b <= BHI; ++b) {
         for (int v = 0; v < 2; ++v)
             if (seg_variant_ok<DT>(b, v)) {
                 const int w = tile_lds_words<Elem<DT>>(b, seg_loge<DT>(b, v));
                 m = w > m ? w : m;
             }
+        if (seg_variant_ok<DT>(b, 3)) {
+            const int w = seg_segments_per_item<DT>(b, 3) * xl::region_words(b);
+            m = w > m ? w : m;
+        }
+    }
     return m;
 }
 
@@ -192,6 +201,29 @@ __device__ __attribute__((noinline)) void seg_tile_item(global_ptr<const typenam
                         lp + ((C::NTRANS && E::kPayload) ? group * C::REGION : 0));
 }
 
+// One item of the cross-lane variant: `count` segments of bucket B, segment
+// i of the item on the lanes of array slot i (lane mapping of the batched
+// cross-lane kernel), each slot with its own LDS region.
+template <int DT, int B>
+__device__ __attribute__((noinline)) void seg_xlane_item(global_ptr<const typename Elem<DT>::Storage> gin,
+                                                         global_ptr<typename Elem<DT>::Storage> gout,
+                                                         global_ptr<const int64_t> goff, global_ptr<const int> glist,
+                                                         int first, int count, typename Elem<DT>::Key* lk) {
+    using E = Elem<DT>;
+    using S = typename E::Storage;
+    constexpr int LOGT = B - xl::kLogE;
+    constexpr int APW = 64 >> LOGT;
+    constexpr int REGION = xl::region_words(B);
+    const int tid = int(threadIdx.x);
+    const int p = tid & 63;
+    const int t = xl::lane_t(p, LOGT);
+    const int slot = (tid >> 6) * APW + xl::lane_g(p, LOGT);
+    int64_t b0 = 0;
+    int len = 0;
+    if (slot < count) seg_range<DT>((const int64_t*)goff, ((const int*)glist)[first + slot], b0, len);
+    xlane_sort<E, B, kXlaneWindow>((const S*)gin + b0, (S*)gout + b0, len, t, lk + slot * REGION);
+}
+
 // One item of the LDS-resident variant: the single segment list[first].
 template <int DT, int B>
 __device__ __attribute__((noinline)) void seg_lds_item(global_ptr<const typename Elem<DT>::Storage> gin,
@@ -220,6 +252,12 @@ __device__ __forceinline__ void seg_bucket(int variant, const typename Elem<DT>:
         if (variant == 1) {
             seg_tile_item<DT, B, 1>(to_global(in), to_global(out), to_global(off), to_global(list), first, count,
                                     lk, lp);
+            return;
+        }
+    }
+    if constexpr (seg_variant_ok<DT>(B, 3)) {
+        if (variant == 3) {
+            seg_xlane_item<DT, B>(to_global(in), to_global(out), to_global(off), to_global(list), first, count, lk);
             return;
         }
     }

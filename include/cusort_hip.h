@@ -179,7 +179,8 @@ int cs_segmented_sort_pairs(cs_segplan* plan, const void* keys_in, const int32_t
                             int32_t* vals_out, const int64_t* d_offsets, hipStream_t stream);
 int cs_segmented_plan_destroy(cs_segplan* plan);
 /* Work items, persistent grid size, longest segment, and per bucket b < buckets
- * the chosen variant (0: 32-key register tile, 1: 16-key tile, 2: LDS network)
+ * the chosen variant (0: 32-key register tile, 1: 16-key tile, 2: LDS network,
+ * 3: cross-lane register engine for 4-byte keys without payload, W = 32..2048)
  * and its segment count.  Any pointer may be NULL. */
 int cs_segmented_plan_info(const cs_segplan* plan, int32_t* work_items, int32_t* grid, int64_t* max_len,
                            int32_t* variants, int64_t* bucket_segments, int32_t buckets);

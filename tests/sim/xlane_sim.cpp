@@ -217,7 +217,8 @@ struct Wave {
             if (!put_std() || !get_win(ks)) return false;
             for (int j = 4; j >= 0; --j) reg_layer(j, ks);
             if (!put_win(ks) || !get_std()) return false;
-            for (int j = ks - 6; j >= 0; --j) reg_layer(j, ks);
+            for (int L = ks - 11; L >= 0; --L) lane_layer(L, ks);
+            for (int j = std::min(ks - 6, 4); j >= 0; --j) reg_layer(j, ks);
         }
         return true;
     }
@@ -383,7 +384,8 @@ int main(int argc, char** argv) {
     }
     const int logw = std::atoi(argv[1]), ldsk = std::atoi(argv[2]);
     if (argc > 3) return oem_exhaustive() ? 0 : 1;
-    if (logw < 5 || logw > 10) return fail("logw outside 5..10") ? 0 : 1;
+    if (logw < 5 || logw > 11) return fail("logw outside 5..11") ? 0 : 1;
+    if (logw == 11 && ldsk > 11) return fail("2048 positions need the window for stage 11") ? 0 : 1;
     if (!check_lane_map(logw - kLogE)) return 1;
     for (int ks = kLogE + 1; ks <= logw; ++ks)
         if (!stage_merger(logw, ldsk, ks)) return 1;

@@ -49,7 +49,7 @@ def test_random_ragged_all_variants(cs, orc, gpu, dtype):
         check_seg(cs, orc, gpu, dt, data, off, flags=cs.SEG_NO_GRAPH)
         check_seg(cs, orc, gpu, dt, data, off, plain=True)
     data = orc.generate(dt, orc.D_UNIFORM, int(off[-1]), int(off[-1]), seed=3)
-    for variant in (0, 1, 2):
+    for variant in (0, 1, 2, 3):   # 3: cross-lane engine, 4-byte keys without payload, buckets 5..11
         for b in range(16):
             cs.segmented_set_variant(dt, b, variant)
         try:
@@ -100,7 +100,9 @@ def test_empty_and_degenerate(cs, orc, gpu):
 def test_tuner_picks_a_valid_variant(cs, orc, gpu):
     cs.segmented_tune(cs.I32, 2)
     for b in range(1, 14):
-        assert cs.segmented_variant(cs.I32, b) in (0, 1, 2)
+        v = cs.segmented_variant(cs.I32, b)
+        assert v in (0, 1, 2, 3)
+        assert v != 3 or 5 <= b <= 11      # the cross-lane engine covers W = 32..2048
     rng = np.random.default_rng(5)
     off = random_offsets(rng, 3000, 8192)
     data = orc.generate(cs.I32, orc.D_UNIFORM, int(off[-1]), int(off[-1]), seed=9)

@@ -28,7 +28,7 @@ def test_lane_sorter_zero_one_exhaustive(sim):
 
 # (log2 W, first stage through an LDS window): the library's instantiations
 # (windows from stage 9) and the all-register / all-window variants of W = 1024
-CASES = [(w, 9) for w in range(5, 11)] + [(10, 10), (10, 11), (9, 11), (8, 6)]
+CASES = [(w, 9) for w in range(5, 12)] + [(10, 10), (10, 11), (9, 11), (8, 6), (11, 11), (11, 10)]
 
 
 @pytest.mark.parametrize("logw,ldsk", CASES)
