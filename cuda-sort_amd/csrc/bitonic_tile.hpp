@@ -308,14 +308,18 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
     constexpr int kBufFirst = C::kMultiWave ? C::APB * C::REGION : 0;
 
     Tile<E, LOGW, LOGE> tile;
-    // 16-byte accesses need 16-byte aligned array starts (n*sizeof % 16 == 0).
-    const bool vec_in = ((uintptr_t)src & 15) == 0 && VS == V;
-    const bool vec_out = ((uintptr_t)dst & 15) == 0 && VS == V;
+    // 16-byte accesses of V elements.  Global memory needs only dword
+    // alignment for them, which every element has, so arrays that start
+    // anywhere (ragged segments, partition buckets) use them too; they are
+    // written as memcpy of an element pointer, so the compiler sees the
+    // element's alignment, not the vector's.
+    constexpr bool vec_in = VS == V;
+    constexpr bool vec_out = VS == V;
 
     // ---- load, lane-striped.  When every group of this lane is in range
     // (all lanes of a full array), the loads form one branch-free block, all
-    // in flight together: 16-byte vectors when the array is 16-byte aligned,
-    // element loads otherwise (ragged segments start anywhere).  A lane
+    // in flight together: 16-byte vectors (element loads for the 16-byte
+    // record tilings whose groups are narrower than a vector).  A lane
     // with a group past the end bounds-checks each group.  Stores are split
     // the same way.
     constexpr int NG = NE / VS;
@@ -325,7 +329,8 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
     if (full_in) {
         sfor<0, NG>([&](auto q_) {
             constexpr int q = decltype(q_)::value;
-            const Vec v = *reinterpret_cast<const Vec*>(src + (q * T + t) * VS);
+            Vec v;
+            __builtin_memcpy(&v, src + (q * T + t) * VS, 16);
             Storage tmp[V];
             __builtin_memcpy(tmp, &v, 16);
             sfor<0, VS>([&](auto j_) {
@@ -349,7 +354,8 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
             constexpr int q = decltype(q_)::value;
             const int g = (q * T + t) * VS;
             if (vec_in && g + VS <= valid) {
-                const Vec v = *reinterpret_cast<const Vec*>(src + g);
+                Vec v;
+                __builtin_memcpy(&v, src + g, 16);
                 Storage tmp[V];
                 __builtin_memcpy(tmp, &v, 16);
                 sfor<0, VS>([&](auto j_) {
@@ -390,7 +396,7 @@ __device__ __forceinline__ void tile_sort(const typename E::Storage* src, typena
         });
         Vec v;
         __builtin_memcpy(&v, tmp, 16);
-        *reinterpret_cast<Vec*>(dst + (q * T + t) * VS) = v;
+        __builtin_memcpy(dst + (q * T + t) * VS, &v, 16);
     };
     if (full_out) {
         sfor<0, NG>(store_group);

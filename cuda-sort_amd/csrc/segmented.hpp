@@ -221,7 +221,7 @@ __device__ __attribute__((noinline)) void seg_xlane_item(global_ptr<const typena
     int64_t b0 = 0;
     int len = 0;
     if (slot < count) seg_range<DT>((const int64_t*)goff, ((const int*)glist)[first + slot], b0, len);
-    xlane_sort<E, B, kXlaneWindow>((const S*)gin + b0, (S*)gout + b0, len, t, lk + slot * REGION);
+    xlane_sort<E, B, kXlaneWindow, true>((const S*)gin + b0, (S*)gout + b0, len, t, lk + slot * REGION);
 }
 
 // One item of the LDS-resident variant: the single segment list[first].

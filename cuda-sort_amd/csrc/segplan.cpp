@@ -250,6 +250,12 @@ int run_plan(cs_segplan* p, const void* in, void* out, const int64_t* d_offsets,
 
 namespace cs {
 
+// Variant of a bucket the tuner has not set: the cross-lane engine where it
+// applies (4-byte keys without payload, W = 32..2048; measured on 2^28 int32
+// partition 3.32 -> 3.29 ms and the 100,000-segment ragged batch 0.539 ->
+// 0.537 ms), else the 32-key register tile.
+static int default_variant(const SegKernels& k, int b) { return k.variant_ok(b, 3) ? 3 : 0; }
+
 SegClasses seg_classes(int dtype) {
     const SegKernels k = seg_kernels_for(dtype);
     SegClasses c;
@@ -257,7 +263,7 @@ SegClasses seg_classes(int dtype) {
     c.small_max = k.small_max;
     for (int b = 0; b <= c.maxb; ++b) {
         int v = tuned_variant(dtype, b);
-        if (v < 0 || !k.variant_ok(b, v)) v = 0;
+        if (v < 0 || !k.variant_ok(b, v)) v = default_variant(k, b);
         if (!k.variant_ok(b, v)) v = 2;
         c.variant[b] = v;
         c.spi[b] = k.segments_per_item(b, v);
@@ -301,7 +307,7 @@ int seg_build_items(int dtype, const int64_t* h_off, int64_t segs, const int* fo
     for (int b = maxb; b >= 0; --b) {
         if (bucket_segments) bucket_segments[b] = count[b];
         int v = force_variant ? force_variant[b] : tuned_variant(dtype, b);
-        if (v < 0 || !k.variant_ok(b, v)) v = 0;
+        if (v < 0 || !k.variant_ok(b, v)) v = default_variant(k, b);
         if (!k.variant_ok(b, v)) v = 2;
         if (variants) variants[b] = v;
         if (count[b] == 0) continue;
