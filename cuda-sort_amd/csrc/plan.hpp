@@ -16,7 +16,7 @@ namespace cs {
 // ------------------------------------------------------------------ plan --
 constexpr int kMaxOps = 320;
 // OP_LANESORT: every lane sorts its E slots ascending (Batcher's odd-even
-// merge sort, xl::make_oem(E)); it stands for stages 1..e, whose only
+// merge sort, xl::make_oem_df(E)); it stands for stages 1..e, whose only
 // contract towards stage e+1 is ascending runs of E positions.
 enum : int { OP_REFLECT = 0, OP_CE = 1, OP_TRANSPOSE = 2, OP_LANESORT = 3 };
 
@@ -88,7 +88,7 @@ template <int LOGW, int LOGE, bool LANESORT = false> struct PlanOf {
     static constexpr Plan value = make_plan(LOGW, LOGE, LANESORT);
 };
 template <int N> struct OemOf {
-    static constexpr xl::CxList value = xl::make_oem(N);
+    static constexpr xl::CxList value = xl::make_oem_df(N);
 };
 
 // Logical index of slot 0 of lane t in layout L(h) (e slot bits): the low

@@ -105,6 +105,39 @@ constexpr CxList make_oem(int n) {
                     }
     return l;
 }
+// The same comparators listed depth first (sort the lower half, sort the
+// upper half, merge): the register tile's in-lane opening uses this order
+// (measured on IntAndData N = 129: 38.56 -> 38.18 us); the cross-lane
+// engine keeps the level order above (its 4096 x 1024 batch read 12.04 ->
+// 12.07 us with this one).
+constexpr void oem_merge(CxList& l, int lo, int n, int r) {
+    const int step = r * 2;
+    if (step < n) {
+        oem_merge(l, lo, n, step);
+        oem_merge(l, lo + r, n, step);
+        for (int i = lo + r; i + r < lo + n; i += step) {
+            l.c[l.n].a = i;
+            l.c[l.n].b = i + r;
+            ++l.n;
+        }
+    } else {
+        l.c[l.n].a = lo;
+        l.c[l.n].b = lo + r;
+        ++l.n;
+    }
+}
+constexpr void oem_sort(CxList& l, int lo, int n) {
+    if (n > 1) {
+        oem_sort(l, lo, n / 2);
+        oem_sort(l, lo + n / 2, n / 2);
+        oem_merge(l, lo, n, 1);
+    }
+}
+constexpr CxList make_oem_df(int n) {
+    CxList l{};
+    oem_sort(l, 0, n);
+    return l;
+}
 constexpr CxList kOem32 = make_oem(32);
 static_assert(kOem32.n == 191, "odd-even merge sort of 32 has 191 comparators");
 

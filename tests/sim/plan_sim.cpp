@@ -46,7 +46,7 @@ struct Sim {
         for (int i = 0; i < plan.nops; ++i) {
             const Op& op = plan.ops[i];
             if (op.kind == OP_LANESORT) {
-                const xl::CxList l = xl::make_oem(E);
+                const xl::CxList l = xl::make_oem_df(E);
                 for (int t = 0; t < T; ++t)
                     for (int c = 0; c < l.n; ++c) cx(t, l.c[c].a, l.c[c].b, false);
             } else if (op.kind == OP_REFLECT) {

@@ -17,7 +17,8 @@
 //      store path (LDS pass + striped read-back);
 //   4. every LDS access stays inside its array's region and every read
 //      returns a value written by the same pass.
-// Usage: xlane_sim <logw> <ldsk> [oem]   (oem: run check 1 only)
+// Usage: xlane_sim <logw> <ldsk> [oem|oemdf]   (run check 1 only, on the
+// level-ordered or the depth-first comparator list)
 #include <algorithm>
 #include <array>
 #include <climits>
@@ -25,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "../../cuda-sort_amd/csrc/xlane_plan.hpp"
@@ -53,8 +55,7 @@ static bool fail(const char* what) {
 // the complement of what the ascending sorter makes of the complement of x,
 // so it sorts every input exactly when the ascending one does.
 typedef uint64_t V4 __attribute__((vector_size(32)));
-static bool oem_exhaustive() {
-    const CxList& l = kOem32;
+static bool oem_exhaustive(const CxList& l) {
     V4 low[8];
     for (int i = 0; i < 8; ++i)
         for (int v = 0; v < 4; ++v) {
@@ -383,7 +384,9 @@ int main(int argc, char** argv) {
         return 2;
     }
     const int logw = std::atoi(argv[1]), ldsk = std::atoi(argv[2]);
-    if (argc > 3) return oem_exhaustive() ? 0 : 1;
+    // "oem": the engine's level-ordered list; "oemdf": the depth-first order
+    // of the register tile's in-lane opening (make_oem_df)
+    if (argc > 3) return oem_exhaustive(std::string(argv[3]) == "oemdf" ? make_oem_df(32) : kOem32) ? 0 : 1;
     if (logw < 5 || logw > 11) return fail("logw outside 5..11") ? 0 : 1;
     if (logw == 11 && ldsk > 11) return fail("2048 positions need the window for stage 11") ? 0 : 1;
     if (!check_lane_map(logw - kLogE)) return 1;

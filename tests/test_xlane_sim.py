@@ -21,8 +21,9 @@ def sim(tmp_path_factory):
     return str(exe)
 
 
-def test_lane_sorter_zero_one_exhaustive(sim):
-    r = subprocess.run([sim, "5", "11", "oem"], capture_output=True, text=True, timeout=600)
+@pytest.mark.parametrize("order", ["oem", "oemdf"])   # engine's level order, register tile's depth-first order
+def test_lane_sorter_zero_one_exhaustive(sim, order):
+    r = subprocess.run([sim, "5", "11", order], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
 
 
