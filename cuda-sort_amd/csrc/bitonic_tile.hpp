@@ -83,11 +83,12 @@ struct Tile {
     static constexpr int w = LOGW;
 
     // Elements whose compare-exchange is a compare and selects (records,
-    // 8-byte keys) open with an odd-even merge sort of each lane's slots:
-    // 191 compare-exchanges for 32 slots against the bitonic stages' 240.
-    // 4-byte keys (one v_min + one v_max per pair) keep the bitonic stages,
-    // whose layers carry more independent pairs.
-    static constexpr bool kLaneSort = E::kPayload || sizeof(Key) > 4;
+    // 8-byte keys), and every multi-wave tile, open with an odd-even merge
+    // sort of each lane's slots: 191 compare-exchanges for 32 slots against
+    // the bitonic stages' 240.  Single-wave tiles of 4-byte keys (one v_min +
+    // one v_max per pair) keep the bitonic stages, whose layers carry more
+    // independent pairs (measured faster there).
+    static constexpr bool kLaneSort = E::kPayload || sizeof(Key) > 4 || C::T > 64;
     using P = PlanOf<LOGW, LOGE, kLaneSort>;
 
     Key k[NE];
